@@ -1,0 +1,99 @@
+/*
+ * C ABI of librcmps_hip.so: the phi^4 energy-and-gradient path of RcmpsCore on one MI355X.
+ *
+ * Matrices crossing the boundary are DEVICE pointers to row-major D x D complex128 arrays
+ * (interleaved re, im doubles; element (i, j) at offset 2*(i*D + j)).  The caller owns them.
+ * Scalars come back through host pointers.  Every entry point returns RCMPS_OK (0) or a
+ * status code; rcmps_status_string() names it.  A context owns one HIP stream and its
+ * workspaces; calls on one context are serialised by the caller, distinct contexts are
+ * independent.
+ *
+ * Reference interfaces replaced (AntoineTilloy/RcmpsCore.jl):
+ *   rcmps_state_create        State(R, K) with its stationary state      src/state.jl:40-53,
+ *                             (CPTP Arnoldi / Poisson solve of rho0)      src/transfer.jl:74-115;
+ *                             rcmps_gpu numerics.stationary_state          gpu/src/rcmps_gpu/optimizer/
+ *                                                                          runtime/numerics.py:74-100
+ *   rcmps_fg_phi4             fg_phi4(s, g) -> (energy, gradient)         src/observables.jl:226-257;
+ *                             rcmps_gpu objective contract                 gpu/src/rcmps_gpu/optimizer/
+ *                                                                          runtime/sweep_driver.py:78-118
+ *   rcmps_boundary_environment  boundary_environment(s, a)                src/transfer.jl:122-153
+ *   rcmps_trajectory_samples  the dense-output samples of one trajectory  src/observables.jl:77-96
+ *                             (forward/adjoint trajectory solutions)
+ */
+#ifndef RCMPS_H
+#define RCMPS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RCMPS_OK 0
+#define RCMPS_ERR_MAX_STEPS 1        /* ODE exceeded maxiters attempts */
+#define RCMPS_ERR_STEP_UNDERFLOW 2   /* ODE step below 10 ulp */
+#define RCMPS_ERR_NONFINITE 3        /* nonfinite error estimate or result */
+#define RCMPS_ERR_MISSING_SAMPLES 5  /* trajectory ended before every quadrature node */
+#define RCMPS_ERR_WAIT_LIMIT 7       /* a workgroup waited past the spin limit (kernel aborted) */
+#define RCMPS_ERR_SOLVER 10          /* Krylov or factorisation failure (SolverFailure) */
+#define RCMPS_ERR_ARGUMENT 11        /* invalid argument (ArgumentError) */
+#define RCMPS_ERR_DEVICE 12          /* HIP runtime error */
+#define RCMPS_ERR_UNSUPPORTED 13     /* configuration not built (e.g. D above the kernel limit) */
+
+#define RCMPS_FLAG_NO_BOUNDARY 1     /* research switch of evaluate(...; boundary=false) */
+
+typedef struct rcmps_context rcmps_context;
+typedef struct rcmps_state rcmps_state;
+
+/* IntegratorParameters (src/state.jl:4-10) plus the DOP853 initial step. */
+typedef struct {
+    int n_quad;
+    double max_point;
+    double atol;
+    double rtol;
+    long long maxiters;
+    double initial_step;
+} rcmps_integrator;
+
+typedef struct {
+    int status;
+    int failed_system;                 /* 0..5 = (kinetic, kinetic commutator, phi^4) x (fwd, adj) */
+    long long attempts[6], accepted[6], rejected[6], dense_steps[6], rhs_calls[6];
+    double term_values[3];             /* kinetic1, kinetic2, <:phi^4:> */
+    double trajectory_ms, pullback_ms, boundary_ms, total_ms;
+    int krylov_matvecs;
+    double krylov_residual;
+} rcmps_stats;
+
+int rcmps_context_create(int device, rcmps_context** out);
+void rcmps_context_destroy(rcmps_context* ctx);
+const char* rcmps_status_string(int status);
+/* largest bond dimension the trajectory kernels are built for */
+int rcmps_max_bond_dimension(void);
+
+/* State(R, K): K must be Hermitian (|K - K^+| <= 1e-12 max(1,|K|)); Q = -iK - R^+R/2.  rho0 is
+   solved to steady_tol unless `rho` is given, in which case it is checked (checked_density). */
+int rcmps_state_create(rcmps_context* ctx, int D, const double* R, const double* K, const double* rho,
+                       double steady_tol, rcmps_state** out, rcmps_stats* stats);
+void rcmps_state_destroy(rcmps_state* st);
+/* copy Q (which = 0) or rho0 (which = 1) into a device D x D buffer */
+int rcmps_state_get(rcmps_context* ctx, const rcmps_state* st, int which, double* dst);
+
+/* fg_phi4: energy and Riemannian gradient (metric 2 Re tr(W1^+ W2 (rho0 + metric_reg I))).
+   grad, raw_cov, raw_env may be null.  raw_cov/raw_env are the summed covector and
+   environment before finish_gradient. */
+int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip,
+                  double metric_reg, int flags, double* energy, double* grad, double* raw_cov,
+                  double* raw_env, rcmps_stats* stats);
+
+/* Solve (-L* + |I><rho0|) H = a - tr(rho0 a) I, tr(rho0 H) = 0. */
+int rcmps_boundary_environment(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol,
+                               double* h, rcmps_stats* stats);
+
+/* After rcmps_fg_phi4 on this context: copy the samples of one trajectory system
+   (2 n_quad + 1 points x live channels x D x D, row-major) into a device buffer of
+   `capacity` doubles.  Returns RCMPS_ERR_ARGUMENT if it is too small. */
+int rcmps_trajectory_samples(rcmps_context* ctx, int system, double* dst, long long capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

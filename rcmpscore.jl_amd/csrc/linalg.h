@@ -1,0 +1,35 @@
+// Host-side declarations of the dense complex kernels (linalg.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+struct GemmArgs {
+    int Dp, D, nbatch, opA, opB, alpha_stride;
+    long long strideA, strideB, strideC, strideD;   // in complex elements
+    const double* A;
+    const double* B;
+    double* C;
+    const double* alpha;    // per-batch real weight (index batch*alpha_stride) or null
+    double beta;
+    double alpha_scale;     // scalar factor on the product; 0 means 1
+    const double* addD;     // optional: C += dscale * dalpha_b * D_b
+    const double* dalpha;
+    double dscale;
+    double add_identity;    // optional: C += add_identity * dalpha_b * I (on the D x D block)
+};
+
+__global__ void rc_zgemm_batched(GemmArgs g);
+__global__ void rc_zgemm_reduce_partial(GemmArgs g, double* partial, int chunk);
+__global__ void rc_sum_partials(const double* partial, int nchunks, int n, double scale, double beta, double* C);
+__global__ void rc_lincomb(const double* X, long long stride, const double* coef, int ncoef, int n, double beta,
+                           double* out);
+__global__ void rc_axpby(int n, double ar, double ai, const double* X, double br, double bi, const double* Y,
+                         double* out);
+__global__ void rc_herm(int Dp, int mode, double scale, const double* X, double* out);
+__global__ void rc_sylvester_diag(int Dp, int D, int adjoint, double shift, const double* lam, const double* Y,
+                                  double* Z);
+__global__ void rc_dots(const double* V, long long stride, int n, const double* w, double* out);
+__global__ void rc_subtract_combo(const double* V, long long stride, int nk, const double* h, int n, double* w);
+__global__ void rc_add_combo(const double* V, long long stride, int nk, const double* y, int n, double* x);
+__global__ void rc_trace_prod(int Dp, const double* A, const double* B, double* out);
+__global__ void rc_add_identity(int Dp, int D, const double* cptr, double scale, double* X);
+__global__ void rc_sub_scaled(int n, const double* cptr, double dre, double dim, const double* Z, double* X);

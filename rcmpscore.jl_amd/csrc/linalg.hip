@@ -1,0 +1,248 @@
+// Dense complex FP64 building blocks on MFMA for Dp x Dp matrices (Dp a multiple of 16):
+// strided-batched products with per-batch real weights, deterministic batch reductions
+// (the pullback's sums over quadrature nodes), and the elementwise / Krylov helpers.
+#include "device_common.h"
+#include "linalg.h"
+
+namespace {
+
+// one 16x16 output tile per workgroup, 4 waves split the k-steps; operands read directly
+// into MFMA fragments (op 0: as stored, op 1: conjugate transpose)
+__device__ __forceinline__ cplx load_op(const cplx* M, int Dp, int op, int r, int c) {
+    if (!op) return M[r * Dp + c];
+    cplx z = M[c * Dp + r];
+    return cmake(z.re, -z.im);
+}
+
+__device__ __forceinline__ void tile_mma(cacc& acc, const cplx* A, int opA, const cplx* B, int opB,
+                                         int Dp, int bi, int bj, int wave, int lane) {
+    const int ks_total = Dp / 4;
+    for (int ks = wave; ks < ks_total; ks += 4) {
+        const int kk = 4 * ks + (lane >> 4);
+        cplx a = load_op(A, Dp, opA, 16 * bi + (lane & 15), kk);
+        cplx b = load_op(B, Dp, opB, kk, 16 * bj + (lane & 15));
+        cmfma(acc, a, b);
+    }
+}
+
+__device__ __forceinline__ cplx reduce4(const cacc& acc, cplx* red, int wave, int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave * 256 + (4 * q + (lane >> 4)) * 16 + (lane & 15)] = cacc_get(acc, q);
+    __syncthreads();
+    const int t = threadIdx.x;
+    cplx s = cadd(cadd(cadd(red[t], red[256 + t]), red[512 + t]), red[768 + t]);
+    __syncthreads();
+    return s;
+}
+
+}  // namespace
+
+// C_b = alpha_b op(A_b) op(B_b) + beta C_b ; grid (NB*NB, nbatch)
+__global__ void __launch_bounds__(256) rc_zgemm_batched(GemmArgs g) {
+    __shared__ cplx red[1024];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nb = g.Dp / 16, bi = blockIdx.x / nb, bj = blockIdx.x % nb, batch = blockIdx.y;
+    const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
+    const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
+    cplx* C = reinterpret_cast<cplx*>(g.C) + (long long)batch * g.strideC;
+    cacc acc;
+    cacc_zero(acc);
+    tile_mma(acc, A, g.opA, B, g.opB, g.Dp, bi, bj, wave, lane);
+    cplx v = reduce4(acc, red, wave, lane);
+    const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * (g.alpha_scale != 0.0 ? g.alpha_scale : 1.0);
+    const int r = 16 * bi + (tid >> 4), c = 16 * bj + (tid & 15);
+    cplx out = cscale(al, v);
+    if (g.beta != 0.0) out = cadd(out, cscale(g.beta, C[r * g.Dp + c]));
+    if (g.addD) {
+        const cplx* Dm = reinterpret_cast<const cplx*>(g.addD) + (long long)batch * g.strideD;
+        const double dl = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
+        out = cadd(out, cscale(dl * g.dscale, Dm[r * g.Dp + c]));
+    }
+    if (g.add_identity != 0.0 && r == c && r < g.D) {
+        const double il = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
+        out.re += il * g.add_identity;
+    }
+    C[r * g.Dp + c] = out;
+}
+
+// partial[chunk] = sum_{b in chunk} alpha_b op(A_b) op(B_b) ; grid (NB*NB, nchunks)
+__global__ void __launch_bounds__(256) rc_zgemm_reduce_partial(GemmArgs g, double* partial, int chunk) {
+    __shared__ cplx red[1024];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nb = g.Dp / 16, bi = blockIdx.x / nb, bj = blockIdx.x % nb;
+    const int b0 = blockIdx.y * chunk, b1 = min(g.nbatch, b0 + chunk);
+    cacc acc;
+    cacc_zero(acc);
+    for (int batch = b0; batch < b1; ++batch) {
+        const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
+        const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
+        const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * (g.alpha_scale != 0.0 ? g.alpha_scale : 1.0);
+        cacc part;
+        cacc_zero(part);
+        tile_mma(part, A, g.opA, B, g.opB, g.Dp, bi, bj, wave, lane);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc.rr[q] += al * part.rr[q];
+            acc.ii[q] += al * part.ii[q];
+            acc.ri[q] += al * part.ri[q];
+            acc.ir[q] += al * part.ir[q];
+        }
+    }
+    cplx v = reduce4(acc, red, wave, lane);
+    const int r = 16 * bi + (tid >> 4), c = 16 * bj + (tid & 15);
+    reinterpret_cast<cplx*>(partial)[(size_t)blockIdx.y * g.Dp * g.Dp + r * g.Dp + c] = v;
+}
+
+// C = beta C + scale * sum_chunks partial (fixed order)
+__global__ void rc_sum_partials(const double* partial, int nchunks, int n, double scale, double beta, double* C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cplx* p = reinterpret_cast<const cplx*>(partial);
+    cplx s = cmake(0, 0);
+    for (int k = 0; k < nchunks; ++k) s = cadd(s, p[(size_t)k * n + i]);
+    cplx* Cc = reinterpret_cast<cplx*>(C);
+    cplx out = cscale(scale, s);
+    if (beta != 0.0) out = cadd(out, cscale(beta, Cc[i]));
+    Cc[i] = out;
+}
+
+// out = sum_k coef[k] * X_k   (X_k = X + k*stride), n complex elements; coef real
+__global__ void rc_lincomb(const double* X, long long stride, const double* coef, int ncoef, int n,
+                           double beta, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cplx* x = reinterpret_cast<const cplx*>(X);
+    cplx s = cmake(0, 0);
+    for (int k = 0; k < ncoef; ++k) s = cadd(s, cscale(coef[k], x[(long long)k * stride + i]));
+    cplx* o = reinterpret_cast<cplx*>(out);
+    if (beta != 0.0) s = cadd(s, cscale(beta, o[i]));
+    o[i] = s;
+}
+
+// elementwise: out = a*X + b*Y (complex scalars)
+__global__ void rc_axpby(int n, double ar, double ai, const double* X, double br, double bi, const double* Y,
+                         double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cplx* x = reinterpret_cast<const cplx*>(X);
+    const cplx* y = reinterpret_cast<const cplx*>(Y);
+    cplx v = cmul(cmake(ar, ai), x[i]);
+    if (Y) v = cadd(v, cmul(cmake(br, bi), y[i]));
+    reinterpret_cast<cplx*>(out)[i] = v;
+}
+
+// out = (X + X^+)/2 scaled; or out = X^+ (mode 1); Dp x Dp
+__global__ void rc_herm(int Dp, int mode, double scale, const double* X, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Dp * Dp) return;
+    const int r = i / Dp, c = i % Dp;
+    const cplx* x = reinterpret_cast<const cplx*>(X);
+    cplx a = x[r * Dp + c], bt = cconj(x[c * Dp + r]);
+    cplx v = mode == 0 ? cscale(0.5 * scale, cadd(a, bt)) : cscale(scale, bt);
+    reinterpret_cast<cplx*>(out)[i] = v;
+}
+
+// Z_ij = -Y_ij / den_ij   (den = lam_i + conj(lam_j) - shift, or conj(lam_i) + lam_j - shift)
+__global__ void rc_sylvester_diag(int Dp, int D, int adjoint, double shift, const double* lam, const double* Y,
+                                  double* Z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Dp * Dp) return;
+    const int r = i / Dp, c = i % Dp;
+    cplx* z = reinterpret_cast<cplx*>(Z);
+    if (r >= D || c >= D) { z[i] = cmake(0, 0); return; }
+    const cplx* l = reinterpret_cast<const cplx*>(lam);
+    cplx den = adjoint ? cadd(cconj(l[r]), l[c]) : cadd(l[r], cconj(l[c]));
+    den.re -= shift;
+    cplx y = reinterpret_cast<const cplx*>(Y)[i];
+    double m = cabs2(den);
+    cplx q = cmake((y.re * den.re + y.im * den.im) / m, (y.im * den.re - y.re * den.im) / m);
+    z[i] = cmake(-q.re, -q.im);
+}
+
+// Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.
+__global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride, int n, const double* w,
+                                               double* out) {
+    __shared__ double sre[256], sim[256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    const cplx* v = reinterpret_cast<const cplx*>(V) + (size_t)k * stride;
+    const cplx* ww = reinterpret_cast<const cplx*>(w);
+    double re = 0, im = 0;
+    for (int i = t; i < n; i += 256) {
+        cplx a = v[i], b = ww[i];
+        re += a.re * b.re + a.im * b.im;
+        im += a.re * b.im - a.im * b.re;
+    }
+    sre[t] = re;
+    sim[t] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) { sre[t] += sre[t + s]; sim[t] += sim[t + s]; }
+        __syncthreads();
+    }
+    if (t == 0) { out[2 * k] = sre[0]; out[2 * k + 1] = sim[0]; }
+}
+
+// w -= sum_k h_k V_k   (h complex, host-provided array in device memory)
+__global__ void rc_subtract_combo(const double* V, long long stride, int nk, const double* h, int n, double* w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cplx* v = reinterpret_cast<const cplx*>(V);
+    const cplx* hh = reinterpret_cast<const cplx*>(h);
+    cplx s = reinterpret_cast<cplx*>(w)[i];
+    for (int k = 0; k < nk; ++k) s = csub(s, cmul(hh[k], v[(size_t)k * stride + i]));
+    reinterpret_cast<cplx*>(w)[i] = s;
+}
+
+// x += sum_k y_k V_k
+__global__ void rc_add_combo(const double* V, long long stride, int nk, const double* y, int n, double* x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cplx* v = reinterpret_cast<const cplx*>(V);
+    const cplx* yy = reinterpret_cast<const cplx*>(y);
+    cplx s = reinterpret_cast<cplx*>(x)[i];
+    for (int k = 0; k < nk; ++k) s = cadd(s, cmul(yy[k], v[(size_t)k * stride + i]));
+    reinterpret_cast<cplx*>(x)[i] = s;
+}
+
+// out[0] = trace(A B) for Dp x Dp (single workgroup)
+__global__ void __launch_bounds__(256) rc_trace_prod(int Dp, const double* A, const double* B, double* out) {
+    __shared__ double sre[256], sim[256];
+    const int t = threadIdx.x;
+    const cplx* a = reinterpret_cast<const cplx*>(A);
+    const cplx* b = reinterpret_cast<const cplx*>(B);
+    double re = 0, im = 0;
+    for (int idx = t; idx < Dp * Dp; idx += 256) {
+        const int i = idx / Dp, k = idx % Dp;
+        cplx p = cmul(a[i * Dp + k], b[k * Dp + i]);
+        re += p.re;
+        im += p.im;
+    }
+    sre[t] = re;
+    sim[t] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) { sre[t] += sre[t + s]; sim[t] += sim[t + s]; }
+        __syncthreads();
+    }
+    if (t == 0) { out[0] = sre[0]; out[1] = sim[0]; }
+}
+
+// X += (c_re + i c_im) I on the D x D block, c read from device memory at cptr (scaled)
+__global__ void rc_add_identity(int Dp, int D, const double* cptr, double scale, double* X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    cplx* x = reinterpret_cast<cplx*>(X);
+    x[i * Dp + i].re += scale * cptr[0];
+    x[i * Dp + i].im += scale * cptr[1];
+}
+
+// X -= (t/den) * Z with t = cptr[0..1] (complex), den complex (host)
+__global__ void rc_sub_scaled(int n, const double* cptr, double dre, double dim, const double* Z, double* X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cplx tt = cmake(cptr[0], cptr[1]);
+    double m = dre * dre + dim * dim;
+    cplx f = cmake((tt.re * dre + tt.im * dim) / m, (tt.im * dre - tt.re * dim) / m);
+    cplx* x = reinterpret_cast<cplx*>(X);
+    x[i] = csub(x[i], cmul(f, reinterpret_cast<const cplx*>(Z)[i]));
+}

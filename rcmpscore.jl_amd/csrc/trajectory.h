@@ -1,0 +1,62 @@
+// Shared host/device description of the persistent DOP853 trajectory launch.
+#pragma once
+#include <stdint.h>
+
+#define RC_NSYS 6           // (kinetic, kinetic commutator, phi^4) x (forward, adjoint)
+#define RC_MAX_LIVE 4
+#define RC_NSTAGE_BUF 16    // one buffer per stage slot of a step
+#define RC_REC 16           // doubles per workgroup record: e5, e3, 12 trace partials, spare
+
+// status codes (shared with the C-ABI): 1-5 as the DOP853 controller of the reference GPU
+// package (device_control_backend.py:23-27), 7 = a workgroup waited past the spin limit.
+#define RC_ST_OK 0
+#define RC_ST_MAX_STEPS 1
+#define RC_ST_STEP_UNDERFLOW 2
+#define RC_ST_NONFINITE 3
+#define RC_ST_MISSING_SAMPLES 5
+#define RC_ST_WAIT_LIMIT 7
+
+struct TrajTask {
+    int sys, ch, b, c;
+    int adj;
+    int edge_src;       // live channel (same system) whose level-1 tile feeds the insertion, -1 none
+    int edge_from_E;    // 1: the insertion reads the E buffer (X * [Q,R]) of edge_src
+    int has_E;          // compute E = X * C at level 1 (kinetic commutator, adjoint channel 2)
+    int trace_kind;     // 0 none; 1: Re sum X conj(O) over the tile; 2: Re diag of the level-1 tile
+    int pad0;
+    double edge_coef;
+    double trace_coef;
+    const double* fixed;     // Dp x Dp complex (row-major, interleaved) or null
+    const double* trace_op;  // O for trace kind 1
+};
+
+struct TrajSys {
+    int nlive, has_scalar, adj, first_task, ntasks, pad0;
+    double n_elems;          // nlive * D^2 + has_scalar: denominator of the RMS norm
+    double* Y;               // [16][nlive][Dp*Dp] complex
+    double* P;               // [16][nlive][Dp*Dp] complex
+    double* E;               // [16][Dp*Dp] complex or null
+    double* rec;             // [2][ntasks][RC_REC]
+    uint32_t* flagY;         // [nlive][NB*NB]
+    uint32_t* flagP;         // [nlive][NB*NB]
+    uint32_t* flagE;         // [NB*NB]
+    uint32_t* flagR;         // [ntasks]
+    double* samples;         // [npoints][nlive][Dp*Dp] complex
+    double* out;             // [16]
+};
+
+struct TrajParams {
+    int D, Dp, npoints, pad0;
+    double a, atol, rtol, h0;
+    long long max_steps, spin_limit;
+    const double* points;
+    const double *Q, *Qh, *R, *Rh, *C;
+    const double* cheb;
+    const TrajTask* tasks;
+    uint32_t* abort_flag;
+    TrajSys sys[RC_NSYS];
+};
+
+// out[] layout per system
+enum { RC_OUT_STATUS = 0, RC_OUT_ATTEMPTS, RC_OUT_ACCEPTED, RC_OUT_REJECTED, RC_OUT_DENSE,
+       RC_OUT_RHS, RC_OUT_SCALAR, RC_OUT_T, RC_OUT_NEXTPOINT };

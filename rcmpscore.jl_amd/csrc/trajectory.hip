@@ -1,0 +1,562 @@
+// Persistent dataflow DOP853 for the six reduced transfer-ODE systems of fg_phi4
+// (kinetic, kinetic commutator, phi^4) x (forward, adjoint), FP64 on MFMA.
+//
+// Equations (reduced channels of src/observables.jl:37-63; Lindblad maps src/transfer.jl:2-19):
+//   forward  k(X) = jac(t) [Q X + X Q^+ + (R X) R^+] + s(t) [insertions + fixed]
+//   adjoint  k(X) = jac(t) [Q^+ X + X Q + R^+ (X R)] + s(t) [insertions + fixed]
+// with jac = 2|t| and s(t) = 2|t| J(t^2).  Integrator and controller: adaptive DOP853 with the
+// error norm h e5 / sqrt((e5 + 0.01 e3) N), factor min(10, 0.9 err^-1/8), rejection factor
+// max(0.2, .), no step across t = 0, 7th-order dense output at the Gauss nodes, as the reference
+// GPU package (gpu/src/rcmps_gpu/optimizer/runtime/ode_backend.py:500-586,
+// device_control_backend.py:30-47).  The FSAL stage is evaluated only after acceptance.
+//
+// Work decomposition: one workgroup per (system, live channel, 16x16 output tile).  A stage is
+// two levels: level 1 forms the tile of P = R X (forward) or X R (adjoint) and publishes it;
+// level 2 forms Q X + X Q^+ (resp. Q^+ X + X Q) while neighbours' P tiles arrive, then adds
+// P R^+ (resp. R^+ P), the insertion terms and the next stage input.  Workgroups never meet at
+// a grid barrier: each waits on per-tile event flags of the tiles it reads; data written for
+// other workgroups uses agent-coherent stores and loads.  The step-end error norm is a
+// deterministic all-gather of per-workgroup records, so every workgroup of a system takes the
+// same controller decision.  Every wait is bounded by a spin limit and watches an abort word.
+#include "device_common.h"
+#include "trajectory.h"
+
+namespace {
+
+template <int NB>
+struct Lay {
+    static constexpr int Dp = 16 * NB;
+    static constexpr int RS = Dp + 1;                  // padded row stride of row blocks
+    static constexpr int ROWBLK = 16 * RS;
+    static constexpr int COLBLK = Dp * 16;
+    static constexpr int BLK = ROWBLK > COLBLK ? ROWBLK : COLBLK;
+    static constexpr int XR = 0, XC = BLK, PB = 2 * BLK, K0 = 3 * BLK, K1 = 4 * BLK, K2 = 5 * BLK,
+                         K3 = 6 * BLK, K4 = 7 * BLK, RED = 8 * BLK;
+    static constexpr int NCPLX = 8 * BLK + 1024;
+    static constexpr int NDOUBLE_SCRATCH = 64;
+    static constexpr int BYTES = NCPLX * 16 + NDOUBLE_SCRATCH * 8;
+};
+
+struct Ctl {
+    uint32_t* abort_flag;
+    long long spin_limit;
+};
+
+// Wave 0 polls up to 64 flags (lane l: flags[l] >= target); all waves then meet.  Returns false
+// when the abort word is set or the spin limit is passed (then the abort word is set).
+__device__ bool wait_flags(const Ctl& ctl, uint32_t* const* addr_lane, int n, uint32_t target,
+                           int* s_abort) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid < 64) {
+        uint32_t* a = lane < n ? addr_lane[lane] : nullptr;
+        long long spins = 0;
+        int bad = 0;
+        while (true) {
+            bool ok = a ? (ld_flag(a) >= target) : true;
+            if (__all(ok)) break;
+            if (++spins > ctl.spin_limit || ld_flag(ctl.abort_flag)) { bad = 1; break; }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (lane == 0) {
+            if (bad) st_flag(ctl.abort_flag, 1u);
+            *s_abort = bad;
+        }
+    }
+    __syncthreads();
+    return *s_abort == 0;
+}
+
+// All threads issued their coherent stores; make them visible, then raise the flag.
+__device__ __forceinline__ void publish(uint32_t* flag, uint32_t value) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        st_flag(flag, value);
+    }
+}
+
+// A fragment from a row block M[16][RS]: lane l, k-step ks -> M[l%16][4ks + l/16]
+template <int NB>
+__device__ __forceinline__ cplx fragA(const cplx* M, int ks, int lane) {
+    return M[(lane & 15) * Lay<NB>::RS + 4 * ks + (lane >> 4)];
+}
+// B fragment from a column block N[Dp][16]: N[4ks + l/16][l%16]
+__device__ __forceinline__ cplx fragB(const cplx* N, int ks, int lane) {
+    return N[(4 * ks + (lane >> 4)) * 16 + (lane & 15)];
+}
+
+// acc += A(row block) * B(col block) over this wave's share of the k-steps
+template <int NB>
+__device__ __forceinline__ void tile_product(cacc& acc, const cplx* A, const cplx* B, int wave, int lane) {
+#pragma unroll
+    for (int ks = wave; ks < 4 * NB; ks += 4) cmfma(acc, fragA<NB>(A, ks, lane), fragB(B, ks, lane));
+}
+
+// Sum the four waves' partial tiles; thread t receives element (t/16, t%16).
+__device__ __forceinline__ cplx reduce_tile(const cacc& acc, cplx* red, int wave, int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave * 256 + (4 * q + (lane >> 4)) * 16 + (lane & 15)] = cacc_get(acc, q);
+    __syncthreads();
+    const int t = threadIdx.x;
+    cplx s = red[t];
+    s = cadd(s, red[256 + t]);
+    s = cadd(s, red[512 + t]);
+    s = cadd(s, red[768 + t]);
+    __syncthreads();
+    return s;
+}
+
+// Deterministic block sum (same inputs give the same bits in every workgroup).
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double s = ((scratch[0] + scratch[1]) + scratch[2]) + scratch[3];
+    __syncthreads();
+    return s;
+}
+
+// Load a row block (rows 16b.., all Dp columns) of a global Dp x Dp matrix into M[16][RS].
+template <int NB, bool COH>
+__device__ __forceinline__ void load_rowblk(cplx* M, const double* g, uint32_t gbytes, int b) {
+    constexpr int Dp = Lay<NB>::Dp;
+    const int t = threadIdx.x;
+    if (COH) {
+        __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
+#pragma unroll
+        for (int it = 0; it < NB; ++it) {
+            int idx = it * 256 + t, i = idx / Dp, k = idx % Dp;
+            M[i * Lay<NB>::RS + k] = ld_coh(r, (uint32_t)(((16 * b + i) * Dp + k) * 16));
+        }
+    } else {
+        const cplx* gc = reinterpret_cast<const cplx*>(g);
+#pragma unroll
+        for (int it = 0; it < NB; ++it) {
+            int idx = it * 256 + t, i = idx / Dp, k = idx % Dp;
+            M[i * Lay<NB>::RS + k] = gc[(16 * b + i) * Dp + k];
+        }
+    }
+}
+
+// Load a column block (all rows, columns 16c..) into N[Dp][16].
+template <int NB, bool COH>
+__device__ __forceinline__ void load_colblk(cplx* N, const double* g, uint32_t gbytes, int c) {
+    constexpr int Dp = Lay<NB>::Dp;
+    const int t = threadIdx.x;
+    if (COH) {
+        __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
+#pragma unroll
+        for (int it = 0; it < NB; ++it) {
+            int idx = it * 256 + t, k = idx / 16, j = idx % 16;
+            N[k * 16 + j] = ld_coh(r, (uint32_t)((k * Dp + 16 * c + j) * 16));
+        }
+    } else {
+        const cplx* gc = reinterpret_cast<const cplx*>(g);
+#pragma unroll
+        for (int it = 0; it < NB; ++it) {
+            int idx = it * 256 + t, k = idx / 16, j = idx % 16;
+            N[k * 16 + j] = gc[k * Dp + 16 * c + j];
+        }
+    }
+}
+
+}  // namespace
+
+template <int NB>
+__global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __restrict__ pp) {
+    using L = Lay<NB>;
+    constexpr int Dp = L::Dp, NT = NB * NB;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx* sm = reinterpret_cast<cplx*>(smem_raw);
+    cplx* Xr = sm + L::XR;
+    cplx* Xc = sm + L::XC;
+    cplx* Pb = sm + L::PB;
+    cplx* Kl1 = sm + L::K0;   // level-1 constant: R row block (fwd) / R col block (adj)
+    cplx* Kq = sm + L::K1;    // Q row block (fwd) / Q^+ row block (adj)
+    cplx* Kx = sm + L::K2;    // Q^+ col block (fwd) / Q col block (adj)
+    cplx* Kp = sm + L::K3;    // R^+ col block (fwd) / R^+ row block (adj)
+    cplx* Kc = sm + L::K4;    // C col block (adj channel with E)
+    cplx* red = sm + L::RED;
+    double* scratch = reinterpret_cast<double*>(sm + L::NCPLX);
+    __shared__ int s_abort;
+    __shared__ uint32_t* s_addr[64];
+    __shared__ double s_tot[16];
+
+    const TrajParams& p = *pp;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ei = tid >> 4, ej = tid & 15;
+    const TrajTask tk = p.tasks[blockIdx.x];
+    const TrajSys S = p.sys[tk.sys];
+    const int b = tk.b, c = tk.c, ch = tk.ch, adj = tk.adj;
+    const int row = 16 * b + ei, col = 16 * c + ej;
+    const bool real_el = row < p.D && col < p.D;
+    const Ctl ctl{p.abort_flag, p.spin_limit};
+    const uint32_t mat_bytes = (uint32_t)(Dp * Dp * 16);
+    const uint32_t stage_stride = (uint32_t)S.nlive * mat_bytes;
+    const uint32_t ybytes = RC_NSTAGE_BUF * stage_stride;
+    const __amdgpu_buffer_rsrc_t rY = rc_rsrc(S.Y, ybytes);
+    const __amdgpu_buffer_rsrc_t rP = rc_rsrc(S.P, ybytes);
+    const __amdgpu_buffer_rsrc_t rE = rc_rsrc(S.E, S.E ? RC_NSTAGE_BUF * mat_bytes : 0);
+    const uint32_t el_off = (uint32_t)((row * Dp + col) * 16);
+    const uint32_t el_off_t = (uint32_t)((col * Dp + row) * 16);   // transposed position
+
+    // constant blocks
+    if (!adj) {
+        load_rowblk<NB, false>(Kl1, p.R, 0, b);
+        load_rowblk<NB, false>(Kq, p.Q, 0, b);
+        load_colblk<NB, false>(Kx, p.Qh, 0, c);
+        load_colblk<NB, false>(Kp, p.Rh, 0, c);
+    } else {
+        load_colblk<NB, false>(Kl1, p.R, 0, c);
+        load_rowblk<NB, false>(Kq, p.Qh, 0, b);
+        load_colblk<NB, false>(Kx, p.Q, 0, c);
+        load_rowblk<NB, false>(Kp, p.Rh, 0, b);
+        if (tk.has_E) load_colblk<NB, false>(Kc, p.C, 0, c);
+    }
+    cplx fixed_e = cmake(0, 0), trop_e = cmake(0, 0);
+    if (tk.fixed) fixed_e = reinterpret_cast<const cplx*>(tk.fixed)[row * Dp + col];
+    if (tk.trace_kind == 1) trop_e = reinterpret_cast<const cplx*>(tk.trace_op)[row * Dp + col];
+    __syncthreads();
+
+    uint32_t* flagY = S.flagY + ch * NT;
+    uint32_t* flagP = S.flagP + ch * NT;
+    const int my_tile = b * NB + c;
+
+    // ------------------------------------------------------------------------------------
+    // one right-hand-side evaluation on the stage input published in slot si with event ev
+    // ------------------------------------------------------------------------------------
+    auto stage = [&](int si, uint32_t ev, double ts, cplx& k_out, double& tr_out) -> bool {
+        const double jac = 2.0 * fabs(ts);
+        const double src = rc_source(ts, p.cheb);
+        // wait for own channel's column block c and row block b
+        if (tid < 64) {
+            int n = 0;
+            if (lane < NB) s_addr[lane] = flagY + lane * NB + c;                    // (lane, c)
+            if (lane < NB) s_addr[NB + lane] = flagY + b * NB + lane;               // (b, lane)
+            (void)n;
+        }
+        __syncthreads();
+        if (!wait_flags(ctl, s_addr, 2 * NB, ev, &s_abort)) return false;
+        const double* Ych = S.Y + ((size_t)si * S.nlive + ch) * Dp * Dp * 2;
+        load_rowblk<NB, true>(Xr, Ych, mat_bytes, b);
+        load_colblk<NB, true>(Xc, Ych, mat_bytes, c);
+        __syncthreads();
+        // level 1
+        cacc acc;
+        cacc_zero(acc);
+        if (!adj) tile_product<NB>(acc, Kl1, Xc, wave, lane);
+        else tile_product<NB>(acc, Xr, Kl1, wave, lane);
+        cplx Pe = reduce_tile(acc, red, wave, lane);
+        const uint32_t poff = (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes;
+        st_coh(rP, poff + el_off, Pe);
+        if (tk.has_E) {
+            cacc_zero(acc);
+            tile_product<NB>(acc, Xr, Kc, wave, lane);
+            cplx Ee = reduce_tile(acc, red, wave, lane);
+            st_coh(rE, (uint32_t)si * mat_bytes + el_off, Ee);
+        }
+        double trv = 0.0;
+        if (tk.trace_kind == 2 && b == c && ei == ej && real_el) trv = Pe.re;
+        if (tk.trace_kind == 1 && real_el) {
+            cplx x = Xr[ei * L::RS + col];
+            trv = x.re * trop_e.re + x.im * trop_e.im;   // Re(x conj(o))
+        }
+        publish(flagP + my_tile, ev);
+        if (tk.has_E) publish(S.flagE + my_tile, ev);
+        // level 2a
+        cacc_zero(acc);
+        tile_product<NB>(acc, Kq, Xc, wave, lane);
+        tile_product<NB>(acc, Xr, Kx, wave, lane);
+        // wait for the P tiles of this channel and the insertion source tiles
+        if (tid < 64) {
+            if (lane < NB) s_addr[lane] = adj ? (flagP + lane * NB + c) : (flagP + b * NB + lane);
+            if (lane == 0 && tk.edge_src >= 0) {
+                uint32_t* fs = tk.edge_from_E ? S.flagE : (S.flagP + tk.edge_src * NT);
+                s_addr[NB] = fs + b * NB + c;
+                s_addr[NB + 1] = fs + c * NB + b;
+            }
+        }
+        __syncthreads();
+        const int nw = NB + (tk.edge_src >= 0 ? 2 : 0);
+        if (!wait_flags(ctl, s_addr, nw, ev, &s_abort)) return false;
+        const double* Pch = S.P + ((size_t)si * S.nlive + ch) * Dp * Dp * 2;
+        if (!adj) load_rowblk<NB, true>(Pb, Pch, mat_bytes, b);
+        else load_colblk<NB, true>(Pb, Pch, mat_bytes, c);
+        cplx ins = cmake(0, 0);
+        if (tk.edge_src >= 0) {
+            cplx s1, s2;
+            if (tk.edge_from_E) {
+                s1 = ld_coh(rE, (uint32_t)si * mat_bytes + el_off);
+                s2 = ld_coh(rE, (uint32_t)si * mat_bytes + el_off_t);
+            } else {
+                uint32_t so = (uint32_t)si * stage_stride + (uint32_t)tk.edge_src * mat_bytes;
+                s1 = ld_coh(rP, so + el_off);
+                s2 = ld_coh(rP, so + el_off_t);
+            }
+            ins = cscale(tk.edge_coef, cadd(s1, cconj(s2)));
+        }
+        __syncthreads();
+        if (!adj) tile_product<NB>(acc, Pb, Kp, wave, lane);
+        else tile_product<NB>(acc, Kp, Pb, wave, lane);
+        cplx Le = reduce_tile(acc, red, wave, lane);
+        cplx k = cadd(cscale(jac, Le), cscale(src, cadd(ins, fixed_e)));
+        if (!real_el) k = cmake(0, 0);
+        k_out = k;
+        if (tk.trace_kind) tr_out = tk.trace_coef * src * block_sum(trv, scratch);
+        else tr_out = 0.0;
+        return true;
+    };
+
+    auto publish_Y = [&](int si, uint32_t ev, cplx v) {
+        st_coh(rY, (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes + el_off, v);
+        publish(flagY + my_tile, ev);
+    };
+
+    // ------------------------------------------------------------------------------------
+    // integrator: a state machine over stage slots with a single stage call site.  Stage
+    // derivatives are folded into running combinations as they arrive, so no per-stage array
+    // is indexed at run time:  accA[s] = sum_j A[s][j] k_j (stage inputs), accB (solution),
+    // acc5/acc3 (error estimators), accD[r] (dense output).
+    // ------------------------------------------------------------------------------------
+    const double a = p.a;
+    cplx accA[RC_NSTAGES_EXT], accB, acc5, acc3, accD[4];
+    cplx y = cmake(0, 0), ynew = cmake(0, 0), k0 = cmake(0, 0), k12 = cmake(0, 0);
+    __shared__ double s_trp[RC_NSTAGES_EXT];
+
+    auto absorb = [&](int j, cplx kj) {
+#pragma unroll
+        for (int s = 1; s < RC_NSTAGES_EXT; ++s)
+            if (s > j) accA[s] = cadd(accA[s], cscale(RC_A[s][j], kj));
+        if (j < RC_NSTAGES) {
+            accB = cadd(accB, cscale(RC_B[j], kj));
+            acc5 = cadd(acc5, cscale(RC_E5[j], kj));
+            acc3 = cadd(acc3, cscale(RC_E3[j], kj));
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accD[r] = cadd(accD[r], cscale(RC_D[r][j], kj));
+    };
+    auto reset_acc = [&]() {
+#pragma unroll
+        for (int s = 0; s < RC_NSTAGES_EXT; ++s) accA[s] = cmake(0, 0);
+        accB = acc5 = acc3 = cmake(0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accD[r] = cmake(0, 0);
+        absorb(0, k0);
+    };
+    auto select_acc = [&](int s) -> cplx {
+        cplx v = accA[1];
+#pragma unroll
+        for (int q = 2; q < RC_NSTAGES_EXT; ++q)
+            if (q == s) v = accA[q];
+        return v;
+    };
+
+    double ysc = 0.0, ysc_new = 0.0;
+    double t = -a, h = fmin(p.h0, 2.0 * a), t_new = t, endpoint = 0.0, factor = 1.0;
+    uint32_t ev = 0;
+    long long attempts = 0, accepted = 0, rejected = 0, dense = 0;
+    int next_point = 0, rej = 0, status = RC_ST_OK;
+    bool alive = true;
+    const __amdgpu_buffer_rsrc_t rS = rc_rsrc(S.samples, (uint32_t)((size_t)p.npoints * S.nlive * mat_bytes));
+    const int my_rank = blockIdx.x - S.first_task;
+
+    int slot = 0;
+    cplx Yin = y;
+    double ts = t;
+    while (true) {
+        ++ev;
+        publish_Y(slot, ev, Yin);
+        cplx kn;
+        double trn;
+        if (!(alive = stage(slot, ev, ts, kn, trn))) break;
+        if (tid == 0) s_trp[slot] = trn;
+        int next = -1;   // -1: start a new step attempt
+        if (slot == 0) {
+            k0 = kn;
+        } else if (slot < RC_NSTAGES - 1) {
+            absorb(slot, kn);
+            next = slot + 1;
+        } else if (slot == RC_NSTAGES - 1) {
+            absorb(slot, kn);
+            ynew = cadd(y, cscale(h, accB));
+            double e5p = 0.0, e3p = 0.0;
+            if (real_el) {
+                double sc = p.atol + p.rtol * fmax(cabsd(y), cabsd(ynew));
+                e5p = cabs2(cmake(acc5.re / sc, acc5.im / sc));
+                e3p = cabs2(cmake(acc3.re / sc, acc3.im / sc));
+            }
+            e5p = block_sum(e5p, scratch);
+            e3p = block_sum(e3p, scratch);
+            // record for the all-gather: e5, e3, trace partials of k_0..k_11
+            const uint32_t par = (uint32_t)(attempts & 1);
+            if (tid < RC_REC) {
+                double v = 0.0;
+                if (tid == 0) v = e5p;
+                else if (tid == 1) v = e3p;
+                else if (tid < 2 + RC_NSTAGES) v = s_trp[tid - 2];
+                st_coh_f64(S.rec + ((size_t)par * S.ntasks + my_rank) * RC_REC + tid, v);
+            }
+            publish(S.flagR + my_rank, (uint32_t)attempts);
+            if (tid < 64) s_addr[lane] = S.flagR + (lane < S.ntasks ? lane : 0);
+            __syncthreads();
+            if (!(alive = wait_flags(ctl, s_addr, S.ntasks, (uint32_t)attempts, &s_abort))) break;
+            if (tid < 64) {
+#pragma unroll
+                for (int q = 0; q < 2 + RC_NSTAGES; ++q) {
+                    double v = 0.0;
+                    if (lane < S.ntasks) v = ld_coh_f64(S.rec + ((size_t)par * S.ntasks + lane) * RC_REC + q);
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                    if (lane == 0) s_tot[q] = v;
+                }
+            }
+            __syncthreads();
+            double e5 = s_tot[0], e3 = s_tot[1];
+            ysc_new = ysc;
+            if (S.has_scalar) {
+                double sb = 0.0, s5 = 0.0, s3 = 0.0;
+#pragma unroll
+                for (int j = 0; j < RC_NSTAGES; ++j) {
+                    sb += RC_B[j] * s_tot[2 + j];
+                    s5 += RC_E5[j] * s_tot[2 + j];
+                    s3 += RC_E3[j] * s_tot[2 + j];
+                }
+                ysc_new = ysc + h * sb;
+                double sc = p.atol + p.rtol * fmax(fabs(ysc), fabs(ysc_new));
+                e5 += (s5 / sc) * (s5 / sc);
+                e3 += (s3 / sc) * (s3 / sc);
+            }
+            __syncthreads();
+            const double den = e5 + 0.01 * e3;
+            const double error = den > 0.0 ? h * e5 / sqrt(den * S.n_elems) : 0.0;
+            if (!isfinite(error)) { status = RC_ST_NONFINITE; break; }
+            factor = error > 0.0 ? fmin(10.0, 0.9 * pow(error, -1.0 / 8.0)) : 10.0;
+            if (error > 1.0) {
+                ++rejected;
+                rej = 1;
+                h *= fmax(0.2, factor);
+                next = -1;
+            } else {
+                ++accepted;
+                t_new = (h == endpoint - t) ? endpoint : t + h;
+                next = RC_NSTAGES;   // FSAL stage
+            }
+        } else if (slot == RC_NSTAGES) {
+            k12 = kn;
+            absorb(slot, kn);
+            if (next_point < p.npoints && p.points[next_point] <= t_new) {
+                ++dense;
+                next = RC_NSTAGES + 1;
+            } else {
+                next = -2;   // finish the accepted step
+            }
+        } else if (slot < RC_NSTAGES_EXT - 1) {
+            absorb(slot, kn);
+            next = slot + 1;
+        } else {
+            absorb(slot, kn);
+            const cplx d0 = csub(ynew, y);
+            const cplx d1 = csub(cscale(h, k0), d0);
+            const cplx d2 = csub(cscale(2.0, d0), cscale(h, cadd(k12, k0)));
+            cplx dd[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dd[r] = cscale(h, accD[r]);
+            while (next_point < p.npoints && p.points[next_point] <= t_new) {
+                const double pt = p.points[next_point];
+                cplx v;
+                if (pt == t_new) {
+                    v = ynew;
+                } else {
+                    const double x = (pt - t) / h, xm = 1.0 - x;
+                    cplx r = cscale(x, dd[3]);
+                    r = cscale(xm, cadd(r, dd[2]));
+                    r = cscale(x, cadd(r, dd[1]));
+                    r = cscale(xm, cadd(r, dd[0]));
+                    r = cscale(x, cadd(r, d2));
+                    r = cscale(xm, cadd(r, d1));
+                    r = cscale(x, cadd(r, d0));
+                    v = cadd(r, y);
+                }
+                st_coh(rS, (uint32_t)next_point * stage_stride + (uint32_t)ch * mat_bytes + el_off, v);
+                ++next_point;
+            }
+            next = -2;
+        }
+        if (next == -2) {
+            // accepted step complete: FSAL and state update
+            y = ynew;
+            k0 = k12;
+            if (tid == 0) s_trp[0] = s_trp[RC_NSTAGES];
+            t = t_new;
+            ysc = ysc_new;
+            h *= rej ? fmin(1.0, factor) : factor;
+            rej = 0;
+            next = -1;
+        }
+        if (next == -1) {
+            // start an attempt from (t, y, k0) with step h
+            if (!(t < a)) break;
+            endpoint = t < 0.0 ? 0.0 : a;
+            h = fmin(h, endpoint - t);
+            if (!(h > 10.0 * fabs(nextafter(t, INFINITY) - t))) { status = RC_ST_STEP_UNDERFLOW; break; }
+            if (attempts >= p.max_steps) { status = RC_ST_MAX_STEPS; break; }
+            ++attempts;
+            reset_acc();
+            next = 1;
+        }
+        slot = next;
+        if (slot == RC_NSTAGES) {
+            Yin = ynew;
+            ts = t + h;
+        } else {
+            Yin = cadd(y, cscale(h, select_acc(slot)));
+            ts = t + RC_C[slot] * h;
+        }
+    }
+    if (!alive) status = RC_ST_WAIT_LIMIT;
+    if (status == RC_ST_OK && next_point != p.npoints) status = RC_ST_MISSING_SAMPLES;
+    __builtin_amdgcn_s_waitcnt(0);
+    if (my_rank == 0 && tid == 0) {
+        S.out[RC_OUT_STATUS] = status;
+        S.out[RC_OUT_ATTEMPTS] = (double)attempts;
+        S.out[RC_OUT_ACCEPTED] = (double)accepted;
+        S.out[RC_OUT_REJECTED] = (double)rejected;
+        S.out[RC_OUT_DENSE] = (double)dense;
+        S.out[RC_OUT_RHS] = (double)ev;
+        S.out[RC_OUT_SCALAR] = ysc;
+        S.out[RC_OUT_T] = t;
+        S.out[RC_OUT_NEXTPOINT] = next_point;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// host launcher
+// ---------------------------------------------------------------------------------------
+int rc_trajectory_smem_bytes(int nb) {
+    switch (nb) {
+        case 1: return Lay<1>::BYTES;
+        case 2: return Lay<2>::BYTES;
+        case 3: return Lay<3>::BYTES;
+        case 4: return Lay<4>::BYTES;
+        default: return -1;
+    }
+}
+
+hipError_t rc_launch_trajectory(int nb, int nblocks, const TrajParams* dparams, hipStream_t stream) {
+    void* args[] = {(void*)&dparams};
+    const void* fn = nullptr;
+    int bytes = rc_trajectory_smem_bytes(nb);
+    switch (nb) {
+        case 1: fn = (const void*)rc_trajectory<1>; break;
+        case 2: fn = (const void*)rc_trajectory<2>; break;
+        case 3: fn = (const void*)rc_trajectory<3>; break;
+        case 4: fn = (const void*)rc_trajectory<4>; break;
+        default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(256), args, (unsigned)bytes, stream);
+}

@@ -1,0 +1,18 @@
+"""rcmps_gpu on MI355X: fg_phi4 (phi^4 energy and Riemannian gradient of a relativistic
+continuous MPS), its state preparation and optimizer, running on hand-written HIP kernels
+(librcmps_hip.so, C ABI in include/rcmps.h).  Importing this package does not touch the GPU.
+"""
+from ._lib import DeviceControlFailure, SolverFailure
+from .observables import boundary_environment, fg_phi4, fg_phi4_raw, trajectory_samples
+from .optimization import (BackwardLineSearch, metric, optimize_state, precondition, retraction)
+from .state import (IntegratorParameters, ManifoldParameters, ModelParameters, RandomState, RandomStateZ2,
+                    State, random_state_arrays, random_state_z2_arrays)
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "BackwardLineSearch", "DeviceControlFailure", "IntegratorParameters", "ManifoldParameters",
+    "ModelParameters", "RandomState", "RandomStateZ2", "SolverFailure", "State", "boundary_environment",
+    "fg_phi4", "fg_phi4_raw", "metric", "optimize_state", "precondition", "random_state_arrays",
+    "random_state_z2_arrays", "retraction", "trajectory_samples",
+]

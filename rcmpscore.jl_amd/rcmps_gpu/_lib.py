@@ -1,0 +1,129 @@
+"""ctypes binding of librcmps_hip.so (C ABI in include/rcmps.h).
+
+The library is the only compute path: when it is missing or no HIP device is present the
+calls raise instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "librcmps_hip.so"
+
+RCMPS_OK = 0
+STATUS = {
+    0: "success", 1: "max_steps", 2: "step_underflow", 3: "nonfinite_error", 5: "missing_samples",
+    7: "wait_limit", 10: "solver_failure", 11: "invalid_argument", 12: "device_error", 13: "unsupported",
+}
+SYSTEMS = ("kinetic forward", "kinetic adjoint", "kinetic-commutator forward",
+           "kinetic-commutator adjoint", "phi4 forward", "phi4 adjoint")
+FLAG_NO_BOUNDARY = 1
+
+
+class SolverFailure(RuntimeError):
+    """A numerical solve failed (src/transfer.jl:28-31): line searches reject such trials."""
+
+
+class DeviceControlFailure(SolverFailure):
+    """The device DOP853 controller stopped (statuses of device_control_backend.py:23-27)."""
+
+    def __init__(self, status, stats=None):
+        self.status = status
+        self.stats = stats
+        super().__init__(f"device DOP853 failed: {STATUS.get(status, status)}"
+                         + (f" in {SYSTEMS[stats.failed_system]}" if stats is not None
+                            and 0 <= stats.failed_system < 6 else ""))
+
+
+class Integrator(ctypes.Structure):
+    _fields_ = [("n_quad", ctypes.c_int), ("max_point", ctypes.c_double), ("atol", ctypes.c_double),
+                ("rtol", ctypes.c_double), ("maxiters", ctypes.c_longlong),
+                ("initial_step", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("failed_system", ctypes.c_int),
+                ("attempts", ctypes.c_longlong * 6), ("accepted", ctypes.c_longlong * 6),
+                ("rejected", ctypes.c_longlong * 6), ("dense_steps", ctypes.c_longlong * 6),
+                ("rhs_calls", ctypes.c_longlong * 6), ("term_values", ctypes.c_double * 3),
+                ("trajectory_ms", ctypes.c_double), ("pullback_ms", ctypes.c_double),
+                ("boundary_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("krylov_matvecs", ctypes.c_int), ("krylov_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        out["status_name"] = STATUS.get(self.status, str(self.status))
+        return out
+
+
+# C ABI symbols: name -> (restype, argtypes)
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "rcmps_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "rcmps_context_destroy": (None, [_P]),
+    "rcmps_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "rcmps_max_bond_dimension": (ctypes.c_int, []),
+    "rcmps_state_create": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_double, ctypes.POINTER(_P),
+                                          ctypes.POINTER(Stats)]),
+    "rcmps_state_destroy": (None, [_P]),
+    "rcmps_state_get": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
+    "rcmps_fg_phi4": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(Integrator), ctypes.c_double,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_double), _P, _P, _P,
+                                     ctypes.POINTER(Stats)]),
+    "rcmps_boundary_environment": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Stats)]),
+    "rcmps_trajectory_samples": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_longlong]),
+}
+
+_lock = threading.Lock()
+_lib = None
+_contexts = {}
+
+
+def load():
+    """Load the shared library (no device calls)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                                   "(make -C rcmpscore.jl_amd/csrc)")
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def context(device: int = 0):
+    """The per-device context (one HIP stream and its workspaces)."""
+    lib = load()
+    with _lock:
+        ctx = _contexts.get(device)
+        if ctx is None:
+            handle = _P()
+            st = lib.rcmps_context_create(int(device), ctypes.byref(handle))
+            if st != RCMPS_OK:
+                raise RuntimeError(f"rcmps_context_create failed: {STATUS.get(st, st)}")
+            ctx = handle
+            _contexts[device] = ctx
+    return ctx
+
+
+def check(status: int, what: str, stats: Stats | None = None):
+    if status == RCMPS_OK:
+        return
+    if status in (1, 2, 3, 5, 7):
+        raise DeviceControlFailure(status, stats)
+    if status == 10:
+        raise SolverFailure(f"{what}: solver failure")
+    if status in (11,):
+        raise ValueError(f"{what}: invalid argument")
+    if status == 13:
+        raise NotImplementedError(f"{what}: configuration not supported by the built kernels")
+    raise RuntimeError(f"{what}: {STATUS.get(status, status)}")

@@ -1,0 +1,92 @@
+"""fg_phi4: energy and Riemannian gradient of the phi^4 model (src/observables.jl:226-257).
+
+The kinetic terms and the normal-ordered quartic are all evaluated as finite-window ODE
+observables (kinetic_backend=:ode, the Julia default): six trajectories (three terms, forward
+and adjoint) integrated by the persistent HIP DOP853 kernel, the pullback over the Gauss nodes,
+the boundary Poisson correction and the metric solve, all in librcmps_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .state import IntegratorParameters, State
+
+PARALLEL = (None, "serial", "terms", "trajectories", "both")
+
+
+def _integrator(p: IntegratorParameters, initial_step: float = 0.01) -> _lib.Integrator:
+    if p.n_quad < 8 or not (p.max_point > 0 and p.atol > 0 and p.rtol > 0 and p.maxiters > 0):
+        raise ValueError("invalid integration controls")
+    return _lib.Integrator(int(p.n_quad), float(p.max_point), float(p.atol), float(p.rtol),
+                           int(p.maxiters), float(initial_step))
+
+
+def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: IntegratorParameters | None = None,
+                metric_reg: float | None = None):
+    """(energy, gradient, raw covector, raw environment, stats dict) on device tensors."""
+    lib = _lib.load()
+    ctx = _lib.context(s.device)
+    ip = _integrator(integrator or s.integrator_param)
+    reg = s.manifold_param.metric_reg if metric_reg is None else metric_reg
+    if reg < 0:
+        raise ValueError("metric regularizer must be nonnegative")
+    dev = torch.device("cuda", s.device)
+    d = s.D
+    G = torch.empty((d, d), dtype=torch.complex128, device=dev)
+    cov = torch.empty_like(G)
+    env = torch.empty_like(G)
+    e = ctypes.c_double()
+    stats = _lib.Stats()
+    flags = 0 if boundary else _lib.FLAG_NO_BOUNDARY
+    st = lib.rcmps_fg_phi4(ctx, s.handle, float(g), ctypes.byref(ip), float(reg), flags, ctypes.byref(e),
+                           G.data_ptr(), cov.data_ptr(), env.data_ptr(), ctypes.byref(stats))
+    _lib.check(st, "fg_phi4", stats)
+    return e.value, G, cov, env, stats.as_dict()
+
+
+def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str = "ode", parallel=None,
+            boundary: bool = True, return_stats: bool = False):
+    """fg_phi4(s, g) -> (energy, gradient) with dE[W] = metric(s, gradient, W) for horizontal W.
+
+    gradient=False returns the energy only.  `parallel` is accepted for signature compatibility:
+    the six trajectories always run concurrently in one launch."""
+    if parallel not in PARALLEL:
+        raise ValueError("parallel must be :serial, :terms, :trajectories, or :both")
+    if kinetic_backend != "ode":
+        raise NotImplementedError("only the finite-window ODE kinetic backend is built")
+    energy, G, _, _, stats = fg_phi4_raw(s, g, boundary=boundary)
+    if not gradient:
+        return (energy, stats) if return_stats else energy
+    grad = G.cpu().numpy()
+    return (energy, grad, stats) if return_stats else (energy, grad)
+
+
+def boundary_environment(s: State, a, *, tol: float = 1e-11) -> np.ndarray:
+    """Solve (-L* + |I><rho|) H = a - tr(rho a) I (src/transfer.jl:122-153)."""
+    lib = _lib.load()
+    ctx = _lib.context(s.device)
+    dev = torch.device("cuda", s.device)
+    at = torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.complex128))).to(dev)
+    torch.cuda.synchronize(dev)
+    h = torch.empty_like(at)
+    stats = _lib.Stats()
+    _lib.check(lib.rcmps_boundary_environment(ctx, s.handle, at.data_ptr(), float(tol), h.data_ptr(),
+                                              ctypes.byref(stats)), "boundary_environment", stats)
+    return h.cpu().numpy()
+
+
+def trajectory_samples(s: State, system: int) -> np.ndarray:
+    """Dense-output samples of one system of the last fg_phi4 on this device:
+    (2 n_quad + 1, live channels, D, D)."""
+    lib = _lib.load()
+    ctx = _lib.context(s.device)
+    nlive = (1, 2, 1, 2, 3, 4)[system]
+    npts = 2 * s.integrator_param.n_quad + 1
+    out = torch.empty((npts, nlive, s.D, s.D), dtype=torch.complex128, device=torch.device("cuda", s.device))
+    _lib.check(lib.rcmps_trajectory_samples(ctx, int(system), out.data_ptr(), out.numel() * 2),
+               "trajectory_samples")
+    return out.cpu().numpy()

@@ -1,0 +1,137 @@
+"""Parity of the HIP fg_phi4 path with the CPU oracle, through the C ABI.
+
+Bars: energy within 1e-12 relative and gradient within 1e-10 relative (to max |G|) of the oracle
+on the same state and rho0 (the FP64 parity bar); stationary state and boundary solve within
+their solver tolerances; closed forms and the golden fixture as in tests/test_oracle.py.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import FIXTURES
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def rg():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import rcmps_gpu
+    return rcmps_gpu
+
+
+def make_state(rg, d, seed, scaling, **kw):
+    r, k = rg.random_state_arrays(d, scaling=scaling, seed=seed)
+    return rg.State(r, k, **kw)
+
+
+def test_coherent_state_closed_forms(rg):
+    note("case: coherent D=1 closed forms")
+    s = rg.State(np.array([[0.3 + 0.1j]]), np.zeros((1, 1)))
+    s.integrator_param.max_point = 30.0
+    s.integrator_param.atol = 1e-12
+    s.integrator_param.rtol = 1e-12
+    e, g, st = rg.fg_phi4(s, 1.0, return_stats=True)
+    k1, k2, p4 = st["term_values"]
+    assert k1 == pytest.approx(0.1, abs=1e-12)
+    assert k2 == pytest.approx(0.0, abs=1e-12)
+    assert p4 == pytest.approx((math.sqrt(2) * 0.3) ** 4, abs=1e-12)
+
+
+@pytest.mark.parametrize("d,seed,scaling", [(2, 1, 0.5), (4, 7, 0.3), (8, 42, 0.2), (5, 3, 0.4)])
+def test_stationary_state_matches_oracle(rg, d, seed, scaling):
+    note(f"case: stationary state D={d} seed={seed}")
+    s = make_state(rg, d, seed, scaling)
+    ref = oracle.steady_state(s.Q, s.R, method="dense" if d <= 12 else "cptp")
+    assert np.abs(s.rho - ref).max() < 1e-10
+    np.testing.assert_allclose(s.Q, oracle.canonical_q(s.R, s.K), atol=1e-14)
+
+
+@pytest.mark.parametrize("d,seed,scaling,n_quad", [(4, 7, 0.3, 32), (8, 42, 0.2, 128), (5, 3, 0.4, 64),
+                                                    (16, 11, 0.2, 64)])
+def test_fg_phi4_matches_oracle(rg, d, seed, scaling, n_quad):
+    note(f"case: fg_phi4 D={d} seed={seed} n_quad={n_quad}")
+    s = make_state(rg, d, seed, scaling)
+    s.integrator_param.n_quad = n_quad
+    e, g, st = rg.fg_phi4(s, 2.0, return_stats=True)
+    ref = oracle.fg_phi4(s.R, s.Q, s.rho, 2.0, n_quad=n_quad)
+    note(f"  E={e!r} oracle={ref['energy']!r} steps gpu={st['accepted']} oracle={list(ref['stats'][:, 0])}")
+    assert abs(e - ref["energy"]) <= 1e-12 * abs(ref["energy"])
+    gscale = np.abs(ref["gradient"]).max()
+    assert np.abs(g - ref["gradient"]).max() <= 1e-10 * gscale
+    # the step sequences are the oracle's
+    assert list(st["accepted"]) == list(ref["stats"][:, 0])
+    assert list(st["rejected"]) == list(ref["stats"][:, 1])
+
+
+def test_trajectory_samples_match_oracle(rg):
+    note("case: trajectory samples D=6")
+    s = make_state(rg, 6, 5, 0.35)
+    s.integrator_param.n_quad = 16
+    rg.fg_phi4(s, 1.5)
+    for system in range(6):
+        got = rg.trajectory_samples(s, system)
+        ref, _ = oracle.trajectory(s.R, s.Q, s.rho, system, n_quad=16)
+        nl = got.shape[1]
+        ref_m = ref[:, : nl * 36].reshape(ref.shape[0], nl, 6, 6)
+        scale = np.abs(ref_m).max()
+        assert np.abs(got - ref_m).max() <= 1e-12 * scale, system
+
+
+def test_golden_fixture(rg):
+    note("case: golden fixture D16")
+    d = FIXTURES / "phi4_g2_D16_z2_portable"
+    ld = lambda name: np.fromfile(d / f"{name}.bin", dtype="<c16").reshape((16, 16), order="F")
+    R, K, rho = ld("R"), ld("K"), ld("rho")
+    s = rg.State(R, K, rho=rho)
+    s.integrator_param.n_quad = 192
+    s.integrator_param.atol = 1e-13
+    s.integrator_param.rtol = 1e-12
+    e, g = rg.fg_phi4(s, 2.0)
+    assert abs(e - (-0.15648780475169466)) < 2e-11
+    ref = oracle.fg_phi4(s.R, s.Q, s.rho, 2.0, n_quad=192, atol=1e-13, rtol=1e-12)
+    assert abs(e - ref["energy"]) <= 1e-12 * abs(ref["energy"])
+    assert np.abs(g - ld("gradient")).max() < 2e-8
+
+
+def test_boundary_environment_matches_dense(rg):
+    note("case: boundary environment D=5")
+    s = make_state(rg, 5, 5, 1.0)
+    a = np.random.default_rng(12).standard_normal((5, 5)) * (1 + 1j)
+    a = a + a.conj().T
+    h = rg.boundary_environment(s, a, tol=1e-12)
+    ref = oracle.boundary_environment(s.Q, s.R, s.rho, a, method="dense")
+    np.testing.assert_allclose(h, ref, atol=1e-9)
+
+
+def test_gradient_by_central_differences(rg):
+    """metric(G, W) equals the central difference of E along the retraction (runtests.jl:106-149)."""
+    note("case: central differences D=3")
+    r, k = rg.random_state_arrays(3, scaling=1.0, seed=182)
+    r, k = 0.12 * r, 0.12**2 * k
+    s = rg.State(r, k, steady_tol=1e-13)
+    for p in (s,):
+        p.integrator_param.max_point = 12.0
+        p.integrator_param.atol = 2e-12
+        p.integrator_param.rtol = 2e-12
+    w = np.random.default_rng(7).standard_normal((3, 3)) + 1j * np.random.default_rng(8).standard_normal((3, 3))
+    w /= np.linalg.norm(w)
+    e0, g = rg.fg_phi4(s, 1.2)
+    h = 1e-5
+    vals = []
+    for sign in (1, -1):
+        t, _ = rg.retraction(s, w, sign * h)
+        t = rg.State(t.R, t.K, steady_tol=1e-13)
+        t.integrator_param = s.integrator_param
+        vals.append(rg.fg_phi4(t, 1.2, gradient=False))
+    fd = (vals[0] - vals[1]) / (2 * h)
+    assert rg.metric(s, g, w) == pytest.approx(fd, rel=2e-5, abs=1e-7)
