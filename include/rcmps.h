@@ -61,6 +61,11 @@ typedef struct {
     double trajectory_ms, pullback_ms, boundary_ms, total_ms;
     int krylov_matvecs;
     double krylov_residual;
+    /* per system, workgroup 0 of the trajectory kernel, microseconds spent in: wait for stage
+       inputs, load inputs, level-1 products, level-2 products before the neighbour tiles, wait for
+       neighbour tiles, load them, remaining products and epilogue, integrator bookkeeping,
+       step-end all-gather, whole kernel, publishing the next stage input, unused */
+    double phase_us[6][12];
 } rcmps_stats;
 
 int rcmps_context_create(int device, rcmps_context** out);

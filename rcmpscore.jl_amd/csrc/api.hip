@@ -128,6 +128,7 @@ struct rcmps_context {
     int last_D = 0, last_Dp = 0, last_npts = 0;
     int nlive[RC_NSYS] = {1, 2, 1, 2, 3, 4};
     hipEvent_t ev[8];
+    double clock_mhz = 100.0;   // constant-rate device clock
 };
 
 struct rcmps_state {
@@ -487,6 +488,9 @@ int rcmps_context_create(int device, rcmps_context** out) {
         ck(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         for (auto& e : ctx->ev) ck(hipEventCreate(&e));
         ck(ctx->small.ensure(4096));
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            ctx->clock_mhz = khz / 1000.0;
         ck(ctx->abort_word.ensure(256));
     });
     if (rc) {
@@ -774,7 +778,7 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         size_t flag_words = 0;
         for (int s = 0; s < RC_NSYS; ++s) flag_words += (size_t)2 * nlive[s] * NT + NT + 64;
         ck(ctx->traj_flags.ensure(flag_words * 4));
-        ck(ctx->traj_out.ensure(RC_NSYS * 16 * 8));
+        ck(ctx->traj_out.ensure(RC_NSYS * RC_OUT_STRIDE * 8));
         ck(ctx->traj_E.ensure((size_t)RC_NSTAGE_BUF * mb));
         uint32_t* fl = static_cast<uint32_t*>(ctx->traj_flags.p);
         for (int s = 0; s < RC_NSYS; ++s) {
@@ -799,7 +803,7 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
             Sy.flagE = fl; fl += NT;
             Sy.flagR = fl; fl += 64;
             Sy.samples = ctx->traj_samples[s].d();
-            Sy.out = ctx->traj_out.d() + s * 16;
+            Sy.out = ctx->traj_out.d() + s * RC_OUT_STRIDE;
             const int term = s / 2, adj = s % 2;
             for (int ch = 0; ch < nlive[s]; ++ch)
                 for (int b = 0; b < NB; ++b)
@@ -856,12 +860,12 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         ck(hipMemcpyAsync(ctx->traj_params.p, &P, sizeof(TrajParams), hipMemcpyHostToDevice, ctx->stream));
         ck(hipMemsetAsync(ctx->traj_flags.p, 0, flag_words * 4, ctx->stream));
         ck(hipMemsetAsync(ctx->abort_word.p, 0, 256, ctx->stream));
-        ck(hipMemsetAsync(ctx->traj_out.p, 0, RC_NSYS * 16 * 8, ctx->stream));
+        ck(hipMemsetAsync(ctx->traj_out.p, 0, RC_NSYS * RC_OUT_STRIDE * 8, ctx->stream));
         ck(hipEventRecord(ctx->ev[0], ctx->stream));
         ck(rc_launch_trajectory(NB, (int)tasks.size(), static_cast<const TrajParams*>(ctx->traj_params.p),
                                 ctx->stream));
         ck(hipEventRecord(ctx->ev[1], ctx->stream));
-        std::vector<double> outs(RC_NSYS * 16);
+        std::vector<double> outs(RC_NSYS * RC_OUT_STRIDE);
         ck(hipMemcpyAsync(outs.data(), ctx->traj_out.d(), outs.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
         ck(hipStreamSynchronize(ctx->stream));
         float tms = 0;
@@ -870,18 +874,19 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         S->status = RCMPS_OK;
         S->failed_system = -1;
         for (int s = 0; s < RC_NSYS; ++s) {
-            const double* o = outs.data() + s * 16;
+            const double* o = outs.data() + s * RC_OUT_STRIDE;
             S->attempts[s] = (long long)o[RC_OUT_ATTEMPTS];
             S->accepted[s] = (long long)o[RC_OUT_ACCEPTED];
             S->rejected[s] = (long long)o[RC_OUT_REJECTED];
             S->dense_steps[s] = (long long)o[RC_OUT_DENSE];
             S->rhs_calls[s] = (long long)o[RC_OUT_RHS];
+            for (int q = 0; q < RC_NTIMERS; ++q) S->phase_us[s][q] = o[RC_OUT_TIMERS + q] / ctx->clock_mhz;
             int stt = (int)o[RC_OUT_STATUS];
             if (stt != RC_ST_OK && S->status == RCMPS_OK) { S->status = stt; S->failed_system = s; }
         }
         ctx->last_D = D; ctx->last_Dp = Dp; ctx->last_npts = npts;
         if (S->status != RCMPS_OK) throw Fail{S->status};
-        for (int term = 0; term < 3; ++term) S->term_values[term] = outs[(2 * term) * 16 + RC_OUT_SCALAR];
+        for (int term = 0; term < 3; ++term) S->term_values[term] = outs[(2 * term) * RC_OUT_STRIDE + RC_OUT_SCALAR];
         const double coef[3] = {1.0, 1.0, g};
         double E = 0;
         for (int term = 0; term < 3; ++term) E += coef[term] * S->term_values[term];

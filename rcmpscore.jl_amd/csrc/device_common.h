@@ -33,8 +33,18 @@ __device__ __forceinline__ double cabsd(cplx a) { return hypot(a.re, a.im); }
 #define RC_COHERENT_POLICY 16  // CPol::SC1
 #define RC_BUF_DWORD3 0x00020000
 
+// wave-uniform copies of values the compiler cannot prove uniform (control state, addresses)
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double unid(double x) { return __builtin_bit_cast(double, uni64(__builtin_bit_cast(uint64_t, x))); }
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rc_rsrc(const void* base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, RC_BUF_DWORD3);
+    void* b = reinterpret_cast<void*>(uni64(reinterpret_cast<uint64_t>(base)));
+    return __builtin_amdgcn_make_buffer_rsrc(b, 0, (int)__builtin_amdgcn_readfirstlane(bytes), RC_BUF_DWORD3);
 }
 
 __device__ __forceinline__ cplx ld_coh(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset) {

@@ -42,7 +42,7 @@ struct TrajSys {
     uint32_t* flagE;         // [NB*NB]
     uint32_t* flagR;         // [ntasks]
     double* samples;         // [npoints][nlive][Dp*Dp] complex
-    double* out;             // [16]
+    double* out;             // [32]: status/counters, then phase timers of workgroup 0
 };
 
 struct TrajParams {
@@ -59,4 +59,6 @@ struct TrajParams {
 
 // out[] layout per system
 enum { RC_OUT_STATUS = 0, RC_OUT_ATTEMPTS, RC_OUT_ACCEPTED, RC_OUT_REJECTED, RC_OUT_DENSE,
-       RC_OUT_RHS, RC_OUT_SCALAR, RC_OUT_T, RC_OUT_NEXTPOINT };
+       RC_OUT_RHS, RC_OUT_SCALAR, RC_OUT_T, RC_OUT_NEXTPOINT, RC_OUT_TIMERS = 16 };
+#define RC_OUT_STRIDE 32
+#define RC_NTIMERS 12

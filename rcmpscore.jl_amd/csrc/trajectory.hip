@@ -63,17 +63,17 @@ __device__ bool wait_flags(const Ctl& ctl, uint32_t* const* addr_lane, int n, ui
         }
     }
     __syncthreads();
-    return *s_abort == 0;
+    return uni(*s_abort) == 0;
 }
 
-// All threads issued their coherent stores; make them visible, then raise the flag.
+// Every store another workgroup reads is a device-scope (sc1) store, which completes only once
+// it is coherent at device scope; waiting for all of them (vmcnt 0) before the flag store is
+// therefore a release for that data, and no L2 writeback is needed (nothing read across
+// workgroups is ever written with a non-coherent store).
 __device__ __forceinline__ void publish(uint32_t* flag, uint32_t value) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        st_flag(flag, value);
-    }
+    if (threadIdx.x == 0) st_flag(flag, value);
 }
 
 // A fragment from a row block M[16][RS]: lane l, k-step ks -> M[l%16][4ks + l/16]
@@ -124,20 +124,26 @@ template <int NB, bool COH>
 __device__ __forceinline__ void load_rowblk(cplx* M, const double* g, uint32_t gbytes, int b) {
     constexpr int Dp = Lay<NB>::Dp;
     const int t = threadIdx.x;
+    cplx v[NB];
     if (COH) {
         __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
 #pragma unroll
         for (int it = 0; it < NB; ++it) {
             int idx = it * 256 + t, i = idx / Dp, k = idx % Dp;
-            M[i * Lay<NB>::RS + k] = ld_coh(r, (uint32_t)(((16 * b + i) * Dp + k) * 16));
+            v[it] = ld_coh(r, (uint32_t)(((16 * b + i) * Dp + k) * 16));
         }
     } else {
         const cplx* gc = reinterpret_cast<const cplx*>(g);
 #pragma unroll
         for (int it = 0; it < NB; ++it) {
             int idx = it * 256 + t, i = idx / Dp, k = idx % Dp;
-            M[i * Lay<NB>::RS + k] = gc[(16 * b + i) * Dp + k];
+            v[it] = gc[(16 * b + i) * Dp + k];
         }
+    }
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+        int idx = it * 256 + t, i = idx / Dp, k = idx % Dp;
+        M[i * Lay<NB>::RS + k] = v[it];
     }
 }
 
@@ -146,20 +152,49 @@ template <int NB, bool COH>
 __device__ __forceinline__ void load_colblk(cplx* N, const double* g, uint32_t gbytes, int c) {
     constexpr int Dp = Lay<NB>::Dp;
     const int t = threadIdx.x;
+    cplx v[NB];
     if (COH) {
         __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
 #pragma unroll
         for (int it = 0; it < NB; ++it) {
             int idx = it * 256 + t, k = idx / 16, j = idx % 16;
-            N[k * 16 + j] = ld_coh(r, (uint32_t)((k * Dp + 16 * c + j) * 16));
+            v[it] = ld_coh(r, (uint32_t)((k * Dp + 16 * c + j) * 16));
         }
     } else {
         const cplx* gc = reinterpret_cast<const cplx*>(g);
 #pragma unroll
         for (int it = 0; it < NB; ++it) {
             int idx = it * 256 + t, k = idx / 16, j = idx % 16;
-            N[k * 16 + j] = gc[k * Dp + 16 * c + j];
+            v[it] = gc[k * Dp + 16 * c + j];
         }
+    }
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+        int idx = it * 256 + t, k = idx / 16, j = idx % 16;
+        N[k * 16 + j] = v[it];
+    }
+}
+
+// Row block b and column block c of the same matrix, all loads in flight together.
+template <int NB>
+__device__ __forceinline__ void load_row_col_coh(cplx* M, cplx* N, const double* g, uint32_t gbytes, int b, int c) {
+    constexpr int Dp = Lay<NB>::Dp;
+    const int t = threadIdx.x;
+    __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
+    cplx vr[NB], vc[NB];
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+        int idx = it * 256 + t, i = idx / Dp, k = idx % Dp;
+        vr[it] = ld_coh(r, (uint32_t)(((16 * b + i) * Dp + k) * 16));
+        int kk = idx / 16, j = idx % 16;
+        vc[it] = ld_coh(r, (uint32_t)((kk * Dp + 16 * c + j) * 16));
+    }
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+        int idx = it * 256 + t, i = idx / Dp, k = idx % Dp;
+        M[i * Lay<NB>::RS + k] = vr[it];
+        int kk = idx / 16, j = idx % 16;
+        N[kk * 16 + j] = vc[it];
     }
 }
 
@@ -228,9 +263,20 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     // ------------------------------------------------------------------------------------
     // one right-hand-side evaluation on the stage input published in slot si with event ev
     // ------------------------------------------------------------------------------------
-    auto stage = [&](int si, uint32_t ev, double ts, cplx& k_out, double& tr_out) -> bool {
+    // phase timers (constant-rate clock ticks): wait Y, load X, level 1 + publish P, level 2a,
+    // wait P, load P/insertions, level 2b + epilogue, bookkeeping, gather, total, publish Y
+    uint64_t tacc[RC_NTIMERS];
+#pragma unroll
+    for (int q = 0; q < RC_NTIMERS; ++q) tacc[q] = 0;
+    uint64_t tmark = __builtin_amdgcn_s_memrealtime();
+    const uint64_t tstart = tmark;
+    auto lap = [&](int q) {
+        uint64_t now = __builtin_amdgcn_s_memrealtime();
+        tacc[q] += now - tmark;
+        tmark = now;
+    };
+    auto stage = [&](int si, uint32_t ev, double ts, double src, cplx& k_out, double& tr_out) -> bool {
         const double jac = 2.0 * fabs(ts);
-        const double src = rc_source(ts, p.cheb);
         // wait for own channel's column block c and row block b
         if (tid < 64) {
             int n = 0;
@@ -239,11 +285,13 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             (void)n;
         }
         __syncthreads();
+        lap(7);
         if (!wait_flags(ctl, s_addr, 2 * NB, ev, &s_abort)) return false;
+        lap(0);
         const double* Ych = S.Y + ((size_t)si * S.nlive + ch) * Dp * Dp * 2;
-        load_rowblk<NB, true>(Xr, Ych, mat_bytes, b);
-        load_colblk<NB, true>(Xc, Ych, mat_bytes, c);
+        load_row_col_coh<NB>(Xr, Xc, Ych, mat_bytes, b, c);
         __syncthreads();
+        lap(1);
         // level 1
         cacc acc;
         cacc_zero(acc);
@@ -266,10 +314,13 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         }
         publish(flagP + my_tile, ev);
         if (tk.has_E) publish(S.flagE + my_tile, ev);
+        lap(2);
         // level 2a
         cacc_zero(acc);
         tile_product<NB>(acc, Kq, Xc, wave, lane);
         tile_product<NB>(acc, Xr, Kx, wave, lane);
+        __syncthreads();
+        lap(3);
         // wait for the P tiles of this channel and the insertion source tiles
         if (tid < 64) {
             if (lane < NB) s_addr[lane] = adj ? (flagP + lane * NB + c) : (flagP + b * NB + lane);
@@ -282,6 +333,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         __syncthreads();
         const int nw = NB + (tk.edge_src >= 0 ? 2 : 0);
         if (!wait_flags(ctl, s_addr, nw, ev, &s_abort)) return false;
+        lap(4);
         const double* Pch = S.P + ((size_t)si * S.nlive + ch) * Dp * Dp * 2;
         if (!adj) load_rowblk<NB, true>(Pb, Pch, mat_bytes, b);
         else load_colblk<NB, true>(Pb, Pch, mat_bytes, c);
@@ -299,6 +351,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             ins = cscale(tk.edge_coef, cadd(s1, cconj(s2)));
         }
         __syncthreads();
+        lap(5);
         if (!adj) tile_product<NB>(acc, Pb, Kp, wave, lane);
         else tile_product<NB>(acc, Kp, Pb, wave, lane);
         cplx Le = reduce_tile(acc, red, wave, lane);
@@ -307,12 +360,15 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         k_out = k;
         if (tk.trace_kind) tr_out = tk.trace_coef * src * block_sum(trv, scratch);
         else tr_out = 0.0;
+        lap(6);
         return true;
     };
 
     auto publish_Y = [&](int si, uint32_t ev, cplx v) {
+        lap(7);
         st_coh(rY, (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes + el_off, v);
         publish(flagY + my_tile, ev);
+        lap(10);
     };
 
     // ------------------------------------------------------------------------------------
@@ -322,21 +378,40 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     // acc5/acc3 (error estimators), accD[r] (dense output).
     // ------------------------------------------------------------------------------------
     const double a = p.a;
+    // accA[i] holds sum_j A[f + i][j] k_j for the stage f + i, where f (frame) is the last
+    // absorbed stage: the next stage input is always accA[1], so every index is static.
     cplx accA[RC_NSTAGES_EXT], accB, acc5, acc3, accD[4];
     cplx y = cmake(0, 0), ynew = cmake(0, 0), k0 = cmake(0, 0), k12 = cmake(0, 0);
     __shared__ double s_trp[RC_NSTAGES_EXT];
+    __shared__ double s_src[RC_NSTAGES_EXT];
+    __shared__ double s_cheb[RC_SOURCE_SEGMENTS * (RC_SOURCE_DEGREE + 1)];
+    for (int i = tid; i < RC_SOURCE_SEGMENTS * (RC_SOURCE_DEGREE + 1); i += 256) s_cheb[i] = p.cheb[i];
+    __syncthreads();
+    int frame = 0;
 
-    auto absorb = [&](int j, cplx kj) {
+    // row j of RC_COEF: accA coefficients in the frame of stage j, then B, E5, E3, D[0..3]
+    __shared__ double s_coef[RC_NSTAGES_EXT][RC_COEF_ROW];
+    for (int i = tid; i < RC_NSTAGES_EXT * RC_COEF_ROW; i += 256) (&s_coef[0][0])[i] = (&RC_COEF[0][0])[i];
+    __syncthreads();
+    auto absorb = [&](int j_in, cplx kj) {
+        const double* c = &s_coef[uni(j_in)][0];
+        double cv[22];
 #pragma unroll
-        for (int s = 1; s < RC_NSTAGES_EXT; ++s)
-            if (s > j) accA[s] = cadd(accA[s], cscale(RC_A[s][j], kj));
-        if (j < RC_NSTAGES) {
-            accB = cadd(accB, cscale(RC_B[j], kj));
-            acc5 = cadd(acc5, cscale(RC_E5[j], kj));
-            acc3 = cadd(acc3, cscale(RC_E3[j], kj));
-        }
+        for (int q = 0; q < 22; ++q) cv[q] = c[q];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) accD[r] = cadd(accD[r], cscale(RC_D[r][j], kj));
+        for (int i = 1; i < RC_NSTAGES_EXT; ++i) accA[i] = cadd(accA[i], cscale(cv[i - 1], kj));
+        accB = cadd(accB, cscale(cv[15], kj));
+        acc5 = cadd(acc5, cscale(cv[16], kj));
+        acc3 = cadd(acc3, cscale(cv[17], kj));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accD[r] = cadd(accD[r], cscale(cv[18 + r], kj));
+    };
+    // the frame advances to stage j: shift the pending combinations down by one
+    auto shift_to = [&](int j_in) {
+#pragma unroll
+        for (int i = 1; i < RC_NSTAGES_EXT - 1; ++i) accA[i] = accA[i + 1];
+        accA[RC_NSTAGES_EXT - 1] = cmake(0, 0);
+        frame = uni(j_in);
     };
     auto reset_acc = [&]() {
 #pragma unroll
@@ -344,14 +419,8 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         accB = acc5 = acc3 = cmake(0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) accD[r] = cmake(0, 0);
+        frame = 0;
         absorb(0, k0);
-    };
-    auto select_acc = [&](int s) -> cplx {
-        cplx v = accA[1];
-#pragma unroll
-        for (int q = 2; q < RC_NSTAGES_EXT; ++q)
-            if (q == s) v = accA[q];
-        return v;
     };
 
     double ysc = 0.0, ysc_new = 0.0;
@@ -371,16 +440,19 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         publish_Y(slot, ev, Yin);
         cplx kn;
         double trn;
-        if (!(alive = stage(slot, ev, ts, kn, trn))) break;
+        const double src = slot == 0 ? rc_source(ts, s_cheb) : unid(s_src[slot]);
+        if (!(alive = stage(slot, ev, ts, src, kn, trn))) break;
         if (tid == 0) s_trp[slot] = trn;
         int next = -1;   // -1: start a new step attempt
         if (slot == 0) {
             k0 = kn;
         } else if (slot < RC_NSTAGES - 1) {
             absorb(slot, kn);
+            shift_to(slot);
             next = slot + 1;
         } else if (slot == RC_NSTAGES - 1) {
             absorb(slot, kn);
+            shift_to(slot);
             ynew = cadd(y, cscale(h, accB));
             double e5p = 0.0, e3p = 0.0;
             if (real_el) {
@@ -399,6 +471,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
                 else if (tid < 2 + RC_NSTAGES) v = s_trp[tid - 2];
                 st_coh_f64(S.rec + ((size_t)par * S.ntasks + my_rank) * RC_REC + tid, v);
             }
+            lap(7);
             publish(S.flagR + my_rank, (uint32_t)attempts);
             if (tid < 64) s_addr[lane] = S.flagR + (lane < S.ntasks ? lane : 0);
             __syncthreads();
@@ -414,15 +487,17 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
                 }
             }
             __syncthreads();
-            double e5 = s_tot[0], e3 = s_tot[1];
+            lap(8);
+            double e5 = unid(s_tot[0]), e3 = unid(s_tot[1]);
             ysc_new = ysc;
             if (S.has_scalar) {
                 double sb = 0.0, s5 = 0.0, s3 = 0.0;
 #pragma unroll
                 for (int j = 0; j < RC_NSTAGES; ++j) {
-                    sb += RC_B[j] * s_tot[2 + j];
-                    s5 += RC_E5[j] * s_tot[2 + j];
-                    s3 += RC_E3[j] * s_tot[2 + j];
+                    const double kj = unid(s_tot[2 + j]);
+                    sb += RC_B[j] * kj;
+                    s5 += RC_E5[j] * kj;
+                    s3 += RC_E3[j] * kj;
                 }
                 ysc_new = ysc + h * sb;
                 double sc = p.atol + p.rtol * fmax(fabs(ysc), fabs(ysc_new));
@@ -447,6 +522,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         } else if (slot == RC_NSTAGES) {
             k12 = kn;
             absorb(slot, kn);
+            shift_to(slot);
             if (next_point < p.npoints && p.points[next_point] <= t_new) {
                 ++dense;
                 next = RC_NSTAGES + 1;
@@ -455,6 +531,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             }
         } else if (slot < RC_NSTAGES_EXT - 1) {
             absorb(slot, kn);
+            shift_to(slot);
             next = slot + 1;
         } else {
             absorb(slot, kn);
@@ -505,14 +582,20 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             if (attempts >= p.max_steps) { status = RC_ST_MAX_STEPS; break; }
             ++attempts;
             reset_acc();
+            // source values at every stage time of this attempt (stages 1..15), one lane each
+            if (tid >= 1 && tid < RC_NSTAGES_EXT) s_src[tid] = rc_source(t + RC_C[tid] * h, s_cheb);
+            __syncthreads();
             next = 1;
         }
-        slot = next;
+        slot = uni(next);
+        next_point = uni(next_point);
+        t = unid(t);
+        h = unid(h);
         if (slot == RC_NSTAGES) {
             Yin = ynew;
             ts = t + h;
         } else {
-            Yin = cadd(y, cscale(h, select_acc(slot)));
+            Yin = cadd(y, cscale(h, accA[1]));
             ts = t + RC_C[slot] * h;
         }
     }
@@ -529,6 +612,11 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         S.out[RC_OUT_SCALAR] = ysc;
         S.out[RC_OUT_T] = t;
         S.out[RC_OUT_NEXTPOINT] = next_point;
+        lap(7);
+        tacc[9] = __builtin_amdgcn_s_memrealtime() - tstart;
+        tacc[11] = 0;
+#pragma unroll
+        for (int q = 0; q < RC_NTIMERS; ++q) S.out[RC_OUT_TIMERS + q] = (double)tacc[q];
     }
 }
 

@@ -5,11 +5,16 @@ calls raise instead of falling back to anything else.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import threading
+import weakref
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "_lib" / "librcmps_hip.so"
+import os
+
+# RCMPS_LIB selects an alternative build of the library (e.g. an instrumented one)
+LIB_PATH = Path(os.environ.get("RCMPS_LIB") or Path(__file__).resolve().parent / "_lib" / "librcmps_hip.so")
 
 RCMPS_OK = 0
 STATUS = {
@@ -49,13 +54,17 @@ class Stats(ctypes.Structure):
                 ("rhs_calls", ctypes.c_longlong * 6), ("term_values", ctypes.c_double * 3),
                 ("trajectory_ms", ctypes.c_double), ("pullback_ms", ctypes.c_double),
                 ("boundary_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("krylov_matvecs", ctypes.c_int), ("krylov_residual", ctypes.c_double)]
+                ("krylov_matvecs", ctypes.c_int), ("krylov_residual", ctypes.c_double),
+                ("phase_us", (ctypes.c_double * 12) * 6)]
 
     def as_dict(self):
         out = {}
         for name, _ in self._fields_:
             v = getattr(self, name)
-            out[name] = list(v) if hasattr(v, "__len__") else v
+            if hasattr(v, "__len__"):
+                out[name] = [list(x) if hasattr(x, "__len__") else x for x in v]
+            else:
+                out[name] = v
         out["status_name"] = STATUS.get(self.status, str(self.status))
         return out
 
@@ -81,6 +90,33 @@ SIGNATURES = {
 _lock = threading.Lock()
 _lib = None
 _contexts = {}
+_states = weakref.WeakSet()
+_closed = False
+
+
+def register_state(obj) -> None:
+    """Track a State so that its device memory is released before the runtime shuts down."""
+    _states.add(obj)
+
+
+def closed() -> bool:
+    return _closed
+
+
+@atexit.register
+def _shutdown() -> None:
+    global _closed
+    if _lib is None or _closed:
+        return
+    for s in list(_states):
+        try:
+            s.close()
+        except Exception:
+            pass
+    for ctx in list(_contexts.values()):
+        _lib.rcmps_context_destroy(ctx)
+    _contexts.clear()
+    _closed = True
 
 
 def load():

@@ -89,6 +89,7 @@ class State:
                                     ctypes.byref(handle), ctypes.byref(stats))
         _lib.check(st, "stationary state", stats)
         self._handle = handle
+        _lib.register_state(self)
         self.steady_stats = stats.as_dict()
         q = torch.empty((self.D, self.D), dtype=torch.complex128, device=dev)
         r0 = torch.empty((self.D, self.D), dtype=torch.complex128, device=dev)
@@ -110,7 +111,7 @@ class State:
         return self._handle
 
     def close(self):
-        if self._handle is not None:
+        if self._handle is not None and not _lib.closed():
             try:
                 _lib.load().rcmps_state_destroy(self._handle)
             finally:
