@@ -33,6 +33,20 @@ def workload(D=64, g=2.0, seed=42, scaling=0.2, n_quad=256, max_point=20.0, atol
     return dict(D=D, g=g, seed=seed, scaling=scaling, n_quad=n_quad, max_point=max_point, atol=atol, rtol=rtol)
 
 
+def profiled_traffic(wl):
+    """HBM bytes per trajectory launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
+    this workload (copy-calibrated), or None when no summary matches the workload."""
+    if (wl["D"], wl["n_quad"]) != (64, 256):
+        return None
+    path = ROOT / "profiles" / "rc_trajectory_D64_counters.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(R, Q, rho, wl, gpu_stats, attempts_sample=10):
     """Oracle (FP64 C port) timed on a bounded sample: `attempts_sample` DOP853 attempts of each
     of the six trajectories, run concurrently on six host threads; the full evaluation time is
@@ -74,6 +88,9 @@ def main(argv=None):
     ap.add_argument("--n-quad", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-attempts", type=int, default=10)
+    ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of this many MiB (a known-size kernel that "
+                         "calibrates HBM byte counters in a profile of the same command)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -84,6 +101,13 @@ def main(argv=None):
 
     import rcmps_gpu
 
+    if args.copy_reference > 0:
+        n = args.copy_reference * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     wl = workload(D=args.D, n_quad=args.n_quad)
     r, k = rcmps_gpu.random_state_arrays(wl["D"], scaling=wl["scaling"], seed=wl["seed"])
     t0 = time.perf_counter()
@@ -108,7 +132,7 @@ def main(argv=None):
     avg_traj_s = sum(traj_ms) / len(traj_ms) / 1e3
     achieved = flops / avg_traj_s / 1e12
     roofline = dict(bound="mfma", achieved=achieved, peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP64_MFMA_PEAK_TFLOPS, traffic=None,
+                    frac=achieved / FP64_MFMA_PEAK_TFLOPS, traffic=profiled_traffic(wl),
                     kernel="rc_trajectory<4>", kernel_ms=avg_traj_s * 1e3,
                     algorithmic_tflop_per_launch=flops / 1e12)
     out = {
@@ -129,8 +153,12 @@ def main(argv=None):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s.R, s.Q, s.rho, wl, stats, args.cpu_attempts)
     print(json.dumps(out), flush=True)
+    s.close()
+    torch.cuda.synchronize()
     return 0
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    rc = main()
+    if rc:
+        sys.exit(rc)
