@@ -86,7 +86,7 @@ void quadrature(int n, double max_point, std::vector<double>& pts, std::vector<d
             dp = n * (z * p1 - p0) / (z * z - 1);
             double dz = p1 / dp;
             z -= dz;
-            if (std::fabs(dz) < 1e-17) break;
+            if (std::fabs(dz) <= 4e-16 * std::fabs(z)) break;
         }
         double p0 = 1, p1 = z;
         for (int k = 2; k <= n; ++k) {
@@ -126,6 +126,10 @@ struct rcmps_context {
     // krylov workspaces
     DevBuf kry_V, kry_w, kry_t1, kry_t2, kry_x, kry_b, kry_z, kry_h, kry_tmp;
     int last_D = 0, last_Dp = 0, last_npts = 0;
+    // quadrature cache: nodes (plus endpoint) and the pullback weights on the device
+    int q_n = 0;
+    double q_max_point = -1.0;
+    std::vector<double> q_pts;
     int nlive[RC_NSYS] = {1, 2, 1, 2, 3, 4};
     hipEvent_t ev[8];
     double clock_mhz = 100.0;   // constant-rate device clock
@@ -201,6 +205,16 @@ struct Dev {
     }
     void copy(const double* src, double* dst) {
         ck(hipMemcpyAsync(dst, src, (size_t)n() * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    // X -= (t / den) Z with t = the device complex at t2
+    void sub_scaled(const double* t2, cd den, const double* Z, double* X) {
+        hipLaunchKernelGGL(rc_sub_scaled, eb(), dim3(256), 0, ctx->stream, n(), t2, den.real(), den.imag(), Z, X);
+        ck(hipGetLastError());
+    }
+    // Y = -Y + scale * c I with c the device complex at c2
+    void neg_add_identity(const double* c2, double scale, double* Y) {
+        hipLaunchKernelGGL(rc_neg_add_identity, eb(), dim3(256), 0, ctx->stream, Dp, D, c2, scale, Y);
+        ck(hipGetLastError());
     }
     cd fetch2(const double* d2) {
         double h[2];
@@ -303,12 +317,12 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
     const int n = dv.n();
     ck(ctx->kry_V.ensure((size_t)(m + 1) * n * 16));
     ck(ctx->kry_w.ensure((size_t)n * 16));
-    ck(ctx->kry_h.ensure((size_t)(m + 2) * 16));
+    ck(ctx->kry_h.ensure((size_t)(4 * (m + 1) + 2) * 8 + 64));
     double* V = ctx->kry_V.d();
     double* w = ctx->kry_w.d();
     double* hdev = ctx->kry_h.d();
     ck(hipMemsetAsync(x, 0, (size_t)n * 16, ctx->stream));
-    std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yk(m), hv(m + 2);
+    std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yk(m), hv(2 * (m + 1) + 1);
     std::vector<double> cs(m);
     matvecs = 0;
     bool converged = false;
@@ -330,19 +344,23 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
         for (; k < m; ++k) {
             A(V + (size_t)k * n * 2, w);
             ++matvecs;
-            for (int pass = 0; pass < 2; ++pass) {
-                hipLaunchKernelGGL(rc_dots, dim3(k + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hdev);
-                ck(hipGetLastError());
-                ck(hipMemcpyAsync(hv.data(), hdev, (size_t)(k + 1) * 16, hipMemcpyDeviceToHost, ctx->stream));
-                ck(hipStreamSynchronize(ctx->stream));
-                for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] += hv[i];
-                hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, k + 1,
-                                   hdev, n, w);
-                ck(hipGetLastError());
-            }
-            double hn = dv.norm(w);
+            // classical Gram-Schmidt twice on the device: h1, h2 (k+1 complex each), then <w,w>
+            double* hd1 = hdev;
+            double* hd2 = hdev + 2 * (m + 1);
+            double* hnn = hdev + 4 * (m + 1);
+            hipLaunchKernelGGL(rc_dots, dim3(k + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd1);
+            hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, k + 1, hd1, n, w);
+            hipLaunchKernelGGL(rc_dots, dim3(k + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd2);
+            hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, k + 1, hd2, n, w);
+            hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hnn);
+            hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(k + 1) * n * 2);
+            ck(hipGetLastError());
+            ck(hipMemcpyAsync(hv.data(), hdev, (size_t)(4 * (m + 1) + 2) * 8, hipMemcpyDeviceToHost, ctx->stream));
+            ck(hipStreamSynchronize(ctx->stream));
+            for (int i = 0; i <= k; ++i)
+                H[(size_t)i * m + k] += hv[i] + hv[(m + 1) + i];
+            double hn = std::sqrt(std::max(0.0, hv[2 * (m + 1)].real()));
             H[(size_t)(k + 1) * m + k] = hn;
-            if (hn > 0) dv.axpby(cd(1.0 / hn, 0), w, cd(0, 0), nullptr, V + (size_t)(k + 1) * n * 2);
             for (int i = 0; i < k; ++i) {
                 cd h1 = H[(size_t)i * m + k], h2 = H[(size_t)(i + 1) * m + k];
                 H[(size_t)i * m + k] = cs[i] * h1 + sn[i] * h2;
@@ -599,14 +617,12 @@ int rcmps_state_create(rcmps_context* ctx, int D, const double* R, const double*
             auto pre = [&](const double* y, double* xo) {
                 nojump_apply(dv, st, y, xo, false, t1, t2);
                 dv.trace_prod(st->ident.d(), xo, tr2);
-                cd tr = dv.fetch2(tr2) / ((double)D * den);
-                dv.axpby(cd(1, 0), xo, -tr, z, xo);
+                dv.sub_scaled(tr2, (double)D * den, z, xo);
             };
             auto op = [&](const double* xin, double* yo) {
                 lindblad(dv, st, xin, yo, false, t1);
                 dv.trace_prod(st->ident.d(), xin, tr2);
-                cd tr = dv.fetch2(tr2) / (double)D;
-                dv.axpby(cd(-1, 0), yo, tr, st->ident.d(), yo);
+                dv.neg_add_identity(tr2, 1.0 / D, yo);
             };
             dv.axpby(cd(1.0 / D, 0), st->ident.d(), cd(0, 0), nullptr, ctx->kry_b.d());
             int mv = 0;
@@ -675,14 +691,12 @@ void boundary_solve(rcmps_context* ctx, const rcmps_state* st, const double* a, 
     auto pre = [&](const double* y, double* xo) {
         nojump_apply(dv, st, y, xo, true, t1, t2);
         dv.trace_prod(st->rho.d(), xo, tr2);
-        cd tr = dv.fetch2(tr2) / den;
-        dv.axpby(cd(1, 0), xo, -tr, z, xo);
+        dv.sub_scaled(tr2, den, z, xo);
     };
     auto op = [&](const double* xin, double* yo) {
         lindblad(dv, st, xin, yo, true, t1);
         dv.trace_prod(st->rho.d(), xin, tr2);
-        cd tr = dv.fetch2(tr2);
-        dv.axpby(cd(-1, 0), yo, tr, st->ident.d(), yo);
+        dv.neg_add_identity(tr2, 1.0, yo);
     };
     auto A = [&](const double* v, double* w) { pre(v, tmp2); op(tmp2, w); };
     const double tol_abs = tol * std::max(1.0, nb);
@@ -736,20 +750,26 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         const size_t mb = (size_t)Dp * Dp * 16;
         const size_t mcount = (size_t)Dp * Dp * 2;   // doubles per matrix
         const int nn = 2 * ip->n_quad, npts = nn + 1;
-        std::vector<double> pts, wts;
-        quadrature(ip->n_quad, ip->max_point, pts, wts);
-        pts.push_back(std::sqrt(ip->max_point));
-        std::vector<double> wv(nn), jv(nn);
-        for (int i = 0; i < nn; ++i) {
-            wv[i] = wts[i] * 2 * std::fabs(pts[i]);
-            jv[i] = wts[i] * host_source(pts[i]);
+        if (ctx->q_n != ip->n_quad || ctx->q_max_point != ip->max_point) {
+            std::vector<double> pts, wts;
+            quadrature(ip->n_quad, ip->max_point, pts, wts);
+            pts.push_back(std::sqrt(ip->max_point));
+            std::vector<double> wv(nn), jv(nn);
+            for (int i = 0; i < nn; ++i) {
+                wv[i] = wts[i] * 2 * std::fabs(pts[i]);
+                jv[i] = wts[i] * host_source(pts[i]);
+            }
+            ck(ctx->traj_points.ensure((size_t)npts * 8));
+            ck(ctx->nodes_w.ensure((size_t)nn * 8));
+            ck(ctx->nodes_j.ensure((size_t)nn * 8));
+            ck(hipMemcpyAsync(ctx->traj_points.d(), pts.data(), (size_t)npts * 8, hipMemcpyHostToDevice, ctx->stream));
+            ck(hipMemcpyAsync(ctx->nodes_w.d(), wv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
+            ck(hipMemcpyAsync(ctx->nodes_j.d(), jv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
+            ck(hipStreamSynchronize(ctx->stream));
+            ctx->q_n = ip->n_quad;
+            ctx->q_max_point = ip->max_point;
+            ctx->q_pts = pts;
         }
-        ck(ctx->traj_points.ensure((size_t)npts * 8));
-        ck(ctx->nodes_w.ensure((size_t)nn * 8));
-        ck(ctx->nodes_j.ensure((size_t)nn * 8));
-        ck(hipMemcpyAsync(ctx->traj_points.d(), pts.data(), (size_t)npts * 8, hipMemcpyHostToDevice, ctx->stream));
-        ck(hipMemcpyAsync(ctx->nodes_w.d(), wv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
-        ck(hipMemcpyAsync(ctx->nodes_j.d(), jv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
 
         // fixed insertion sources (6 matrices)
         ck(ctx->traj_fixed.ensure(6 * mb));

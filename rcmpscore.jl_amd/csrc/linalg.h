@@ -33,3 +33,5 @@ __global__ void rc_add_combo(const double* V, long long stride, int nk, const do
 __global__ void rc_trace_prod(int Dp, const double* A, const double* B, double* out);
 __global__ void rc_add_identity(int Dp, int D, const double* cptr, double scale, double* X);
 __global__ void rc_sub_scaled(int n, const double* cptr, double dre, double dim, const double* Z, double* X);
+__global__ void rc_normalize(int n, const double* w, const double* nrm2, double* out);
+__global__ void rc_neg_add_identity(int Dp, int D, const double* cptr, double scale, double* Y);

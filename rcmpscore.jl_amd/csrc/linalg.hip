@@ -246,3 +246,25 @@ __global__ void rc_sub_scaled(int n, const double* cptr, double dre, double dim,
     cplx* x = reinterpret_cast<cplx*>(X);
     x[i] = csub(x[i], cmul(f, reinterpret_cast<const cplx*>(Z)[i]));
 }
+
+// out = w / sqrt(Re nrm2[0]) (nrm2 is a device <w, w>); zero stays zero
+__global__ void rc_normalize(int n, const double* w, const double* nrm2, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = nrm2[0] > 0.0 ? 1.0 / sqrt(nrm2[0]) : 0.0;
+    reinterpret_cast<cplx*>(out)[i] = cscale(s, reinterpret_cast<const cplx*>(w)[i]);
+}
+
+// y = -y + (scale * c) I on the D x D block, c complex in device memory
+__global__ void rc_neg_add_identity(int Dp, int D, const double* cptr, double scale, double* Y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Dp * Dp) return;
+    cplx* y = reinterpret_cast<cplx*>(Y);
+    cplx v = cscale(-1.0, y[i]);
+    const int r = i / Dp, c = i % Dp;
+    if (r == c && r < D) {
+        v.re += scale * cptr[0];
+        v.im += scale * cptr[1];
+    }
+    y[i] = v;
+}
