@@ -3,6 +3,7 @@ continuous MPS), its state preparation and optimizer, running on hand-written HI
 (librcmps_hip.so, C ABI in include/rcmps.h).  Importing this package does not touch the GPU.
 """
 from ._lib import DeviceControlFailure, SolverFailure
+from .io import StateRecord, rcmps_load, rcmps_store, read_portable, write_portable
 from .observables import boundary_environment, fg_phi4, fg_phi4_raw, trajectory_samples
 from .optimization import (BackwardLineSearch, metric, optimize_state, precondition, retraction)
 from .state import (IntegratorParameters, ManifoldParameters, ModelParameters, RandomState, RandomStateZ2,
@@ -14,5 +15,6 @@ __all__ = [
     "BackwardLineSearch", "DeviceControlFailure", "IntegratorParameters", "ManifoldParameters",
     "ModelParameters", "RandomState", "RandomStateZ2", "SolverFailure", "State", "boundary_environment",
     "fg_phi4", "fg_phi4_raw", "metric", "optimize_state", "precondition", "random_state_arrays",
-    "random_state_z2_arrays", "retraction", "trajectory_samples",
+    "random_state_z2_arrays", "retraction", "trajectory_samples", "StateRecord", "rcmps_load", "rcmps_store",
+    "read_portable", "write_portable",
 ]
