@@ -89,6 +89,17 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
                   double metric_reg, int flags, double* energy, double* grad, double* raw_cov,
                   double* raw_env, rcmps_stats* stats);
 
+/* Energy and its Euclidean gradient in (R, K), for Q = -iK - R^+R/2 with rho0 re-solved:
+   dE = Re sum_ij [conj(grad_R_ij) dR_ij + conj(grad_K_ij) dK_ij] for every dR and every
+   Hermitian dK (grad_K is Hermitian), i.e. torch's convention for a real function of complex
+   inputs.  No counterpart in the reference, which returns only the Riemannian gradient of
+   fg_phi4 (src/observables.jl:161-212); it is the same adjoint pullback with the coefficients
+   of dR and dQ kept apart, for callers that differentiate through (R, K) (a
+   torch.autograd.Function).  grad_R or grad_K may be null; when both are, only the energy is
+   computed and only the three forward trajectories run. */
+int rcmps_fg_phi4_rk(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip, int flags,
+                     double* energy, double* grad_R, double* grad_K, rcmps_stats* stats);
+
 /* Solve (-L* + |I><rho0|) H = a - tr(rho0 a) I, tr(rho0 H) = 0. */
 int rcmps_boundary_environment(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol,
                                double* h, rcmps_stats* stats);

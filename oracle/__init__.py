@@ -49,6 +49,8 @@ def lib():
         _lib.or_fg_phi4.argtypes = [ctypes.c_int, p, p, p, ctypes.c_double, ctypes.POINTER(Integrator),
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     p, p, p, p, p, p]
+        _lib.or_fg_phi4_rk.argtypes = [ctypes.c_int, p, p, p, ctypes.c_double, ctypes.POINTER(Integrator),
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int, p, p, p]
         _lib.or_trajectory.argtypes = [ctypes.c_int, p, p, p, ctypes.c_int, ctypes.POINTER(Integrator), p, p]
         _lib.or_system_len.restype = ctypes.c_size_t
         _lib.or_system_len.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -120,6 +122,29 @@ def fg_phi4(R, Q, rho, g, *, n_quad=128, max_point=20.0, atol=1e-11, rtol=1e-10,
     _check(st, "oracle fg_phi4")
     return dict(energy=e.value, gradient=G, covector=cov, environment=env, term_values=terms,
                 stats=stats.reshape(6, 5))
+
+
+def fg_phi4_rk(R, K, g, *, rho=None, n_quad=128, max_point=20.0, atol=1e-11, rtol=1e-10, initial_step=0.01,
+               max_steps=1_000_000, boundary_tol=1e-13, boundary=True, boundary_method="iterative",
+               steady_tol=1e-12, nthreads=6):
+    """(energy, grad_R, grad_K) of the state (R, K): dE = Re sum(conj(grad_R) dR + conj(grad_K) dK)
+    for Hermitian dK.  rho0 is solved by the oracle unless given."""
+    d = len(R)
+    Q = canonical_q(R, K)
+    if rho is None:
+        rho = steady_state(Q, R, tol=steady_tol)
+    R, pr = _c(R)
+    Q, pq = _c(Q)
+    rho, prho = _c(rho)
+    ip = integrator(n_quad, max_point, atol, rtol, initial_step, max_steps)
+    e = ctypes.c_double()
+    GR = np.zeros((d, d), np.complex128)
+    GK = np.zeros((d, d), np.complex128)
+    flags = (0 if boundary else 1) | (2 if boundary_method == "dense" else 0)
+    st = lib().or_fg_phi4_rk(d, pr, pq, prho, float(g), ctypes.byref(ip), float(boundary_tol), flags,
+                             int(nthreads), ctypes.addressof(e), GR.ctypes.data, GK.ctypes.data)
+    _check(st, "oracle fg_phi4_rk")
+    return e.value, GR, GK
 
 
 def system_len(d: int, system: int) -> int:

@@ -716,15 +716,20 @@ static void addtriple(int n, cx* out, const cx* a, int ta, const cx* b, int tb, 
     mm(n, coefficient, tmp, 0, c, tc, 1, out);
 }
 
+/* covector (horizontal directions dQ = -R^+ dR) and environment; when xr, xq are given, also the
+ * separate coefficients of dR and dQ, dvalue = 2 Re tr(xr dR) + 2 Re tr(xq dQ): M and N split as
+ * M = MR - MQ R^+, N = NR - R NQ with MR = sum w x R^+ a, MQ = sum w x a, NR = sum w a R x,
+ * NQ = sum w a x, and xr = (MR + NR^+)/2, xq = (MQ + NQ^+)/2 plus the edge-operator terms. */
 static void pullback(const system_t* fw, const system_t* bw, const cx* fsamples, const cx* bsamples,
                      const double* points, const double* weights, int nnodes,
-                     cx* covector, cx* env, double* value) {
+                     cx* covector, cx* env, double* value, cx* xr, cx* xq) {
     int n = fw->ctx.n;
     size_t m = (size_t)n * n;
     const cx *q = fw->ctx.q, *r = fw->ctx.r;
     spec_t spec = fw->ctx.spec;
     int ch = spec.channels;
     cx *M = cx_alloc(m), *N = cx_alloc(m), *U = cx_alloc(m), *V = cx_alloc(m), *tmp = cx_alloc(m);
+    cx *MR = cx_alloc(m), *MQ = cx_alloc(m), *NR = cx_alloc(m), *NQ = cx_alloc(m);
     cx* rho = cx_alloc(m * ch);
     cx* obs = cx_alloc(m * ch);
     for (int i = 0; i < nnodes; ++i) {
@@ -737,6 +742,12 @@ static void pullback(const system_t* fw, const system_t* bw, const cx* fsamples,
             addtriple(n, M, x, 0, a, 0, r, 1, -w, tmp);
             addtriple(n, N, a, 0, r, 0, x, 0, w, tmp);
             addtriple(n, N, r, 0, a, 0, x, 0, -w, tmp);
+            if (xr) {
+                addtriple(n, MR, x, 0, r, 1, a, 0, w, tmp);
+                mm(n, w, x, 0, a, 0, 1, MQ);
+                addtriple(n, NR, a, 0, r, 0, x, 0, w, tmp);
+                mm(n, w, a, 0, x, 0, 1, NQ);
+            }
         }
         double j = weights[i] * or_source(points[i]);
         for (int e = 0; e < spec.nedges; ++e) {
@@ -744,6 +755,25 @@ static void pullback(const system_t* fw, const system_t* bw, const cx* fsamples,
             if (ed.left) mm(n, j * ed.coefficient, rho + ed.src * m, 0, obs + ed.dst * m, 0, 1, U);
             else mm(n, j * ed.coefficient, obs + ed.dst * m, 0, rho + ed.src * m, 0, 1, V);
         }
+    }
+    if (xr) {
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                size_t ik = (size_t)i * n + k, ki = (size_t)k * n + i;
+                xr[ik] = (MR[ik] + conj(NR[ki])) / 2;
+                xq[ik] = (MQ[ik] + conj(NQ[ki])) / 2;
+            }
+        /* edge operator O with coefficient U + V^+ (left and right insertions) */
+        cx* W = cx_alloc(m);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) W[(size_t)i * n + k] = (U[(size_t)i * n + k] + conj(V[(size_t)k * n + i])) / 2;
+        if (spec.commutator) {   /* O = QR - RQ: dO = dQ R + Q dR - dR Q - R dQ */
+            mm(n, 1, W, 0, q, 0, 1, xr); mm(n, -1, q, 0, W, 0, 1, xr);
+            mm(n, 1, r, 0, W, 0, 1, xq); mm(n, -1, W, 0, r, 0, 1, xq);
+        } else {
+            for (size_t i = 0; i < m; ++i) xr[i] += W[i];
+        }
+        free(W);
     }
     if (spec.commutator) {
         mm(n, 1, U, 0, q, 0, 1, M); mm(n, -1, q, 0, U, 0, 1, M);
@@ -766,6 +796,7 @@ static void pullback(const system_t* fw, const system_t* bw, const cx* fsamples,
     /* value: tr of the terminal forward channel = carried scalar */
     *value = creal(fsamples[(size_t)nnodes * fw->len + fw->len - 1]);
     free(M); free(N); free(U); free(V); free(tmp); free(rho); free(obs);
+    free(MR); free(MQ); free(NR); free(NQ);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -1172,9 +1203,10 @@ void or_rhs_full(int n, const cx* q, const cx* r, int term, int adj, double t, c
  * per-term values, per-system stats (6 x 5 longs).  flags bit0: skip the boundary term;
  * bit1: dense boundary solve.  Returns a status code.
  */
-int or_fg_phi4(int n, const cx* r, const cx* q, const cx* rho, double g, const or_integrator_t* ip,
-               double metric_reg, double boundary_tol, int flags, int nthreads,
-               double* energy, cx* grad, cx* raw_cov, cx* raw_env, double* term_values, long* stats) {
+static int fg_phi4_core(int n, const cx* r, const cx* q, const cx* rho, double g, const or_integrator_t* ip,
+                        double metric_reg, double boundary_tol, int flags, int nthreads,
+                        double* energy, cx* grad, cx* raw_cov, cx* raw_env, double* term_values, long* stats,
+                        cx* grad_r, cx* grad_k) {
     size_t m = (size_t)n * n;
     if (n < 1 || ip->n_quad < 8 || !(ip->max_point > 0) || !(ip->atol > 0) || !(ip->rtol > 0))
         return OR_BAD_ARGUMENT;
@@ -1206,14 +1238,18 @@ int or_fg_phi4(int n, const cx* r, const cx* q, const cx* rho, double g, const o
     cx* env = cx_alloc(m);
     cx* c1 = cx_alloc(m);
     cx* e1 = cx_alloc(m);
+    cx *xr = NULL, *xq = NULL, *xr1 = NULL, *xq1 = NULL;
+    if (grad_r) { xr = cx_alloc(m); xq = cx_alloc(m); xr1 = cx_alloc(m); xq1 = cx_alloc(m); }
     if (!st) {
         for (int term = 0; term < 3; ++term) {
             double v;
             pullback(&sys[2 * term], &sys[2 * term + 1], samples[2 * term], samples[2 * term + 1],
-                     pts, wts, nn, c1, e1, &v);
+                     pts, wts, nn, c1, e1, &v, xr1, xq1);
             if (term_values) term_values[term] = v;
             value += coeff[term] * v;
             for (size_t i = 0; i < m; ++i) { cov[i] += coeff[term] * c1[i]; env[i] += coeff[term] * e1[i]; }
+            if (xr)
+                for (size_t i = 0; i < m; ++i) { xr[i] += coeff[term] * xr1[i]; xq[i] += coeff[term] * xq1[i]; }
         }
         if (raw_cov) memcpy(raw_cov, cov, sizeof(cx) * m);
         if (raw_env) memcpy(raw_env, env, sizeof(cx) * m);
@@ -1225,14 +1261,55 @@ int or_fg_phi4(int n, const cx* r, const cx* q, const cx* rho, double g, const o
             if (!st) {
                 addtriple(n, cov, rho, 0, r, 1, h, 0, 1, tmp);
                 addtriple(n, cov, rho, 0, h, 0, r, 1, -1, tmp);
+                if (xr) {   /* rho0 responds to dQ rho0 + rho0 dQ^+ + dR rho0 R^+ + R rho0 dR^+ */
+                    addtriple(n, xr, rho, 0, r, 1, h, 0, 1, tmp);
+                    mm(n, 1, rho, 0, h, 0, 1, xq);
+                }
             }
             free(h); free(tmp);
         }
-        if (!st) st = metric_solve(n, rho, metric_reg, cov, grad);
+        if (!st && grad) st = metric_solve(n, rho, metric_reg, cov, grad);
+        if (!st && grad_r) {
+            /* dQ = -i dK - (dR^+ R + R^+ dR)/2:  grad_R = 2 (xr - herm(xq) R^+)^+,
+               grad_K = 2 herm(-i xq) = -i (xq - xq^+) */
+            cx* t = cx_alloc(m);
+            cx* hq = cx_alloc(m);
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < n; ++k) hq[(size_t)i * n + k] = (xq[(size_t)i * n + k] + conj(xq[(size_t)k * n + i])) / 2;
+            memcpy(t, xr, sizeof(cx) * m);
+            mm(n, -1, hq, 0, r, 1, 1, t);
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < n; ++k) {
+                    grad_r[(size_t)i * n + k] = 2 * conj(t[(size_t)k * n + i]);
+                    grad_k[(size_t)i * n + k] = -I * (xq[(size_t)i * n + k] - conj(xq[(size_t)k * n + i]));
+                }
+            free(t); free(hq);
+        }
     }
     for (int s = 0; s < 6; ++s) free(samples[s]);
     free(cov); free(env); free(c1); free(e1); free(pts); free(wts); free(comm);
+    free(xr); free(xq); free(xr1); free(xq1);
     return st;
+}
+
+int or_fg_phi4(int n, const cx* r, const cx* q, const cx* rho, double g, const or_integrator_t* ip,
+               double metric_reg, double boundary_tol, int flags, int nthreads,
+               double* energy, cx* grad, cx* raw_cov, cx* raw_env, double* term_values, long* stats) {
+    return fg_phi4_core(n, r, q, rho, g, ip, metric_reg, boundary_tol, flags, nthreads, energy, grad, raw_cov,
+                        raw_env, term_values, stats, NULL, NULL);
+}
+
+/*
+ * Energy and Euclidean gradient in (R, K) for Q = -iK - R^+R/2 with rho0 fixed by (R, K):
+ * dE = Re sum conj(grad_r) dR + Re sum conj(grad_k) dK for Hermitian dK.  The reference has only
+ * the Riemannian gradient (src/observables.jl:161-212); this keeps the dR and dQ coefficients of
+ * the same pullback apart.
+ */
+int or_fg_phi4_rk(int n, const cx* r, const cx* q, const cx* rho, double g, const or_integrator_t* ip,
+                  double boundary_tol, int flags, int nthreads, double* energy, cx* grad_r, cx* grad_k) {
+    if (!grad_r || !grad_k) return OR_BAD_ARGUMENT;
+    return fg_phi4_core(n, r, q, rho, g, ip, 0.0, boundary_tol, flags, nthreads, energy, NULL, NULL, NULL, NULL,
+                        NULL, grad_r, grad_k);
 }
 
 /* Exposed helpers for tests */

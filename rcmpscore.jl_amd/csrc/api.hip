@@ -126,6 +126,7 @@ struct rcmps_context {
     // krylov workspaces
     DevBuf kry_V, kry_w, kry_t1, kry_t2, kry_x, kry_b, kry_z, kry_h, kry_tmp;
     int last_D = 0, last_Dp = 0, last_npts = 0;
+    bool last_adjoint = false;   // the last evaluation integrated the adjoint systems
     // quadrature cache: nodes (plus endpoint) and the pullback weights on the device
     int q_n = 0;
     double q_max_point = -1.0;
@@ -334,8 +335,11 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
     };
     double beta = residual_into_w();
     resid = beta;
+    double beta_prev = INFINITY;
     for (int outer = 0; outer < maxiter && !converged; ++outer) {
         if (beta <= tol_abs) { converged = true; break; }
+        if (beta > 0.9 * beta_prev) break;   // a restart cycle that gains under 10%: stagnation
+        beta_prev = beta;
         dv.axpby(cd(1.0 / beta, 0), w, cd(0, 0), nullptr, V);
         std::fill(H.begin(), H.end(), cd(0, 0));
         std::fill(g.begin(), g.end(), cd(0, 0));
@@ -659,8 +663,10 @@ int rcmps_state_get(rcmps_context* ctx, const rcmps_state* st, int which, double
 namespace {
 
 // boundary Poisson solve on padded device matrices: h solves (-L* + |I><rho|) h = a - tr(rho a) I
-void boundary_solve(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol, double* h,
-                    rcmps_stats* stats) {
+// GMRES aims at `tol`; the solution is accepted when the true residual is within 20 accept_tol
+// (relative to max(1, |b|)), the bar of src/transfer.jl:140-152.
+void boundary_solve(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol, double accept_tol,
+                    double* h, rcmps_stats* stats) {
     const int D = st->D, Dp = st->Dp;
     const size_t mb = (size_t)Dp * Dp * 16;
     Dev dv{ctx, Dp, D};
@@ -710,7 +716,8 @@ void boundary_solve(rcmps_context* ctx, const rcmps_state* st, const double* a, 
     dv.axpby(cd(1, 0), tmp, cd(-1, 0), b, tmp);
     const double true_res = dv.norm(tmp);
     if (stats) { stats->krylov_matvecs = mv; stats->krylov_residual = true_res; }
-    if (!ok || !(true_res <= 20 * tol * std::max(1.0, nb))) throw Fail{RCMPS_ERR_SOLVER};
+    (void)ok;
+    if (!(true_res <= 20 * std::max(tol, accept_tol) * std::max(1.0, nb))) throw Fail{RCMPS_ERR_SOLVER};
 }
 
 }  // namespace
@@ -724,15 +731,15 @@ int rcmps_boundary_environment(rcmps_context* ctx, const rcmps_state* st, const 
         double* ap = ctx->tmpM.d();
         double* hp = ap + (size_t)st->Dp * st->Dp * 2;
         pad_in(ctx, st->D, st->Dp, a, ap);
-        boundary_solve(ctx, st, ap, tol, hp, stats);
+        boundary_solve(ctx, st, ap, tol, tol, hp, stats);
         pad_out(ctx, st->D, st->Dp, hp, h);
         ck(hipStreamSynchronize(ctx->stream));
     });
 }
 
-int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip,
-                  double metric_reg, int flags, double* energy, double* grad, double* raw_cov, double* raw_env,
-                  rcmps_stats* stats) {
+static int fg_phi4_impl(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip,
+                        double metric_reg, int flags, double* energy, double* grad, double* raw_cov,
+                        double* raw_env, double* grad_R, double* grad_K, rcmps_stats* stats) {
     if (!ctx || !st || !ip || !energy) return RCMPS_ERR_ARGUMENT;
     if (ip->n_quad < 8 || !(ip->max_point > 0) || !(ip->atol > 0) || !(ip->rtol > 0) || ip->maxiters < 1 ||
         !(metric_reg >= 0) || !std::isfinite(g))
@@ -789,7 +796,8 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         dv.gemm(0, st->rho.d(), 0, st->Rh.d(), fQF, 1.0, 1.0);
         dv.axpby(cd(4, 0), st->R.d(), cd(4, 0), st->Rh.d(), fQA);         // 4 (R + R^+)
 
-        // systems and tasks
+        // systems and tasks; the energy alone needs only the forward trajectories
+        const bool energy_only = !grad && !raw_cov && !raw_env && !grad_R && !grad_K;
         const int NT = NB * NB;
         const int nlive[RC_NSYS] = {1, 2, 1, 2, 3, 4};
         const int has_scalar[RC_NSYS] = {1, 0, 1, 0, 1, 0};
@@ -807,7 +815,8 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
             Sy.has_scalar = has_scalar[s];
             Sy.adj = s % 2;
             Sy.first_task = (int)tasks.size();
-            Sy.ntasks = nlive[s] * NT;
+            Sy.ntasks = energy_only && s % 2 ? 0 : nlive[s] * NT;
+            if (Sy.ntasks == 0) continue;
             Sy.n_elems = (double)nlive[s] * D * D + has_scalar[s];
             ck(ctx->traj_Y[s].ensure((size_t)RC_NSTAGE_BUF * nlive[s] * mb));
             ck(ctx->traj_P[s].ensure((size_t)RC_NSTAGE_BUF * nlive[s] * mb));
@@ -911,6 +920,8 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         double E = 0;
         for (int term = 0; term < 3; ++term) E += coef[term] * S->term_values[term];
         *energy = E;
+        ctx->last_adjoint = !energy_only;
+        if (energy_only) return;
 
         // ----------------------------------------------------------------------------------
         // pullback (src/observables.jl:161-199) in reduced channels
@@ -929,7 +940,11 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         double* cterm = cov + 7 * mcount;
         double* tmp = cov + 8 * mcount;
         double* tmp2 = cov + 9 * mcount;
-        ck(hipMemsetAsync(cov, 0, 2 * mb, ctx->stream));
+        // dE = 2 Re tr(XR dR) + 2 Re tr(XQ dQ) for any (dR, dQ); covector = XR - XQ R^+
+        double* XR = cov + 10 * mcount;
+        double* XQ = cov + 11 * mcount;
+        ck(hipMemsetAsync(env, 0, mb, ctx->stream));
+        ck(hipMemsetAsync(XR, 0, 2 * mb, ctx->stream));
         const long long node = (long long)Dp * Dp;   // complex elements per matrix
         for (int term = 0; term < 3; ++term) {
             const int sf = 2 * term, sa = sf + 1;
@@ -971,23 +986,18 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
                     g2.alpha = ctx->nodes_w.d(); g2.alpha_stride = 1;
                     dv.reduce(g2, nn, T2, 1.0, 1.0);
                 }
-                // u = sqrt2 (rho sum j A2 + (sum j F2)^+)
+                // u = sqrt2 (rho sum j A2 + (sum j F2)^+), the coefficient of the edge operator
                 dv.gemm(0, st->rho.d(), 0, X2, U, s2);
                 dv.herm(1, s2, tmp2, tmp);
                 dv.axpby(cd(1, 0), U, cd(1, 0), tmp, U);
-                // bulk = T1 - T2 R^+
-                dv.gemm(0, T2, 0, st->Rh.d(), cterm);
-                dv.axpby(cd(1, 0), T1, cd(-1, 0), cterm, cterm);
                 if (term == 0) {
-                    dv.axpby(cd(1, 0), cterm, cd(1, 0), U, cterm);
+                    dv.axpby(cd(1, 0), T1, cd(1, 0), U, T1);           // op = R
                 } else {
-                    // + u Q - Q u - R u R^+ + u R R^+   (commutator pullback, delta Q = -R^+ W)
-                    dv.gemm(0, U, 0, st->Q.d(), cterm, 1.0, 1.0);
-                    dv.gemm(0, st->Q.d(), 0, U, cterm, -1.0, 1.0);
-                    dv.gemm(0, st->R.d(), 0, U, tmp);
-                    dv.gemm(0, tmp, 0, st->Rh.d(), cterm, -1.0, 1.0);
-                    dv.gemm(0, U, 0, st->R.d(), tmp);
-                    dv.gemm(0, tmp, 0, st->Rh.d(), cterm, 1.0, 1.0);
+                    // op = C = QR - RQ: dC = dQ R + Q dR - dR Q - R dQ
+                    dv.gemm(0, U, 0, st->Q.d(), T1, 1.0, 1.0);
+                    dv.gemm(0, st->Q.d(), 0, U, T1, -1.0, 1.0);
+                    dv.gemm(0, st->R.d(), 0, U, T2, 1.0, 1.0);
+                    dv.gemm(0, U, 0, st->R.d(), T2, -1.0, 1.0);
                 }
             } else {
                 // phi^4: sum_k F_k (w R^+ A_k + j k A_{k+1}) - (sum_k w F_k A_k) R^+,  F_1 = rho, A_5 = I
@@ -1015,16 +1025,20 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
                     g2.alpha = ctx->nodes_w.d(); g2.alpha_stride = 1;
                     dv.reduce(g2, nn, T2, 1.0, 1.0);
                 }
-                dv.gemm(0, T2, 0, st->Rh.d(), cterm);
-                dv.axpby(cd(1, 0), T1, cd(-1, 0), cterm, cterm);
             }
             // environment: hermitised adjoint channel 1 at the endpoint
             const double* aend = As + 2 * ((long long)nn * astride);
             dv.herm(0, 1.0, aend, tmp);
             const double c = coef[term];
-            dv.axpby(cd(1, 0), cov, cd(c, 0), cterm, cov);
+            dv.axpby(cd(1, 0), XR, cd(c, 0), T1, XR);
+            dv.axpby(cd(1, 0), XQ, cd(c, 0), T2, XQ);
             dv.axpby(cd(1, 0), env, cd(c, 0), tmp, env);
         }
+        auto covector = [&] {   // horizontal directions dQ = -R^+ W
+            dv.gemm(0, XQ, 0, st->Rh.d(), cterm);
+            dv.axpby(cd(1, 0), XR, cd(-1, 0), cterm, cov);
+        };
+        covector();
         ck(hipEventRecord(ctx->ev[3], ctx->stream));
         if (raw_cov) pad_out(ctx, D, Dp, cov, raw_cov);
         if (raw_env) pad_out(ctx, D, Dp, env, raw_env);
@@ -1032,17 +1046,38 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
         // ----------------------------------------------------------------------------------
         // finish_gradient (src/observables.jl:201-212)
         // ----------------------------------------------------------------------------------
-        if (grad) {
+        if (grad || grad_R || grad_K) {
             if (!(flags & RCMPS_FLAG_NO_BOUNDARY)) {
                 ck(ctx->tmpM.ensure(2 * mb));
                 double* hb = X1;
-                boundary_solve(ctx, st, env, 1e-13, hb, S);
-                // cov += rho R^+ H - rho H R^+
+                // aims two digits below the reference's 1e-11 and accepts at its bar
+                boundary_solve(ctx, st, env, 1e-13, 1e-11, hb, S);
+                // rho0 responds to dL(rho0) = dQ rho0 + rho0 dQ^+ + dR rho0 R^+ + R rho0 dR^+:
+                // XR += rho R^+ H, XQ += rho H  (covector += rho R^+ H - rho H R^+)
                 dv.gemm(0, st->rho.d(), 0, st->Rh.d(), tmp);
-                dv.gemm(0, tmp, 0, hb, cov, 1.0, 1.0);
-                dv.gemm(0, st->rho.d(), 0, hb, tmp);
-                dv.gemm(0, tmp, 0, st->Rh.d(), cov, -1.0, 1.0);
+                dv.gemm(0, tmp, 0, hb, XR, 1.0, 1.0);
+                dv.gemm(0, st->rho.d(), 0, hb, XQ, 1.0, 1.0);
+                covector();
             }
+        }
+        if (grad_R || grad_K) {
+            // Q = -iK - R^+R/2: dQ = -i dK - (dR^+ R + R^+ dR)/2, so
+            // dE = 2 Re tr[(XR - herm(XQ) R^+) dR] + 2 Re tr[herm(-i XQ) dK]; returned as the
+            // gradients G with dE = Re <G, d> = Re sum conj(G_ij) d_ij (K restricted to Hermitian)
+            if (grad_R) {
+                dv.herm(0, 1.0, XQ, tmp2);
+                dv.gemm(0, tmp2, 0, st->Rh.d(), tmp);
+                dv.axpby(cd(1, 0), XR, cd(-1, 0), tmp, tmp);
+                dv.herm(1, 2.0, tmp, tmp2);
+                pad_out(ctx, D, Dp, tmp2, grad_R);
+            }
+            if (grad_K) {
+                dv.herm(1, 1.0, XQ, tmp);
+                dv.axpby(cd(0, -1), XQ, cd(0, 1), tmp, tmp2);
+                pad_out(ctx, D, Dp, tmp2, grad_K);
+            }
+        }
+        if (grad) {
             // G = [(rho + reg I)^{-1} cov]^+ = cov^+ (rho + reg I)^{-1}
             rcmps_state* ms = const_cast<rcmps_state*>(st);
             if (ms->metric_reg_cached != metric_reg) {
@@ -1070,8 +1105,21 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
     return rc;
 }
 
+int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip,
+                  double metric_reg, int flags, double* energy, double* grad, double* raw_cov, double* raw_env,
+                  rcmps_stats* stats) {
+    return fg_phi4_impl(ctx, st, g, ip, metric_reg, flags, energy, grad, raw_cov, raw_env, nullptr, nullptr,
+                        stats);
+}
+
+int rcmps_fg_phi4_rk(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip, int flags,
+                     double* energy, double* grad_R, double* grad_K, rcmps_stats* stats) {
+    return fg_phi4_impl(ctx, st, g, ip, 0.0, flags, energy, nullptr, nullptr, nullptr, grad_R, grad_K, stats);
+}
+
 int rcmps_trajectory_samples(rcmps_context* ctx, int system, double* dst, long long capacity) {
     if (!ctx || system < 0 || system >= RC_NSYS || !dst || ctx->last_Dp == 0) return RCMPS_ERR_ARGUMENT;
+    if (system % 2 && !ctx->last_adjoint) return RCMPS_ERR_ARGUMENT;
     const int D = ctx->last_D, Dp = ctx->last_Dp, nl = ctx->nlive[system];
     const long long need = (long long)ctx->last_npts * nl * D * D * 2;
     if (capacity < need) return RCMPS_ERR_ARGUMENT;

@@ -65,6 +65,29 @@ def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str =
     return (energy, grad, stats) if return_stats else (energy, grad)
 
 
+def fg_phi4_rk(s: State, g: float, *, gradient: bool = True, boundary: bool = True,
+               integrator: IntegratorParameters | None = None):
+    """(energy, grad_R, grad_K, stats): the energy of State(R, K) and its Euclidean gradient in
+    (R, K) as device tensors, dE = Re sum(conj(grad_R) dR + conj(grad_K) dK) for Hermitian dK.
+    gradient=False integrates the forward trajectories only and returns None gradients."""
+    lib = _lib.load()
+    ctx = _lib.context(s.device)
+    ip = _integrator(integrator or s.integrator_param)
+    dev = torch.device("cuda", s.device)
+    GR = GK = None
+    if gradient:
+        GR = torch.empty((s.D, s.D), dtype=torch.complex128, device=dev)
+        GK = torch.empty_like(GR)
+    e = ctypes.c_double()
+    stats = _lib.Stats()
+    flags = 0 if boundary else _lib.FLAG_NO_BOUNDARY
+    st = lib.rcmps_fg_phi4_rk(ctx, s.handle, float(g), ctypes.byref(ip), flags, ctypes.byref(e),
+                              GR.data_ptr() if gradient else None, GK.data_ptr() if gradient else None,
+                              ctypes.byref(stats))
+    _lib.check(st, "fg_phi4_rk", stats)
+    return e.value, GR, GK, stats.as_dict()
+
+
 def boundary_environment(s: State, a, *, tol: float = 1e-11) -> np.ndarray:
     """Solve (-L* + |I><rho|) H = a - tr(rho a) I (src/transfer.jl:122-153)."""
     lib = _lib.load()
