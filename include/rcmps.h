@@ -39,6 +39,11 @@ extern "C" {
 #define RCMPS_ERR_UNSUPPORTED 13     /* configuration not built (e.g. D above the kernel limit) */
 
 #define RCMPS_FLAG_NO_BOUNDARY 1     /* research switch of evaluate(...; boundary=false) */
+/* Split precision: the D x D products of the trajectory right-hand sides run on the INT8 MFMA
+   over exact 8-bit digit slices of the FP64 operands (48-bit operands, exact INT32 sums, FP64
+   recombination); integration state, error control, dense output, pullback, boundary and
+   metric solves stay FP64. */
+#define RCMPS_FLAG_SPLIT 2
 
 typedef struct rcmps_context rcmps_context;
 typedef struct rcmps_state rcmps_state;
@@ -64,7 +69,8 @@ typedef struct {
     /* per system, workgroup 0 of the trajectory kernel, microseconds spent in: wait for stage
        inputs, load inputs, level-1 products, level-2 products before the neighbour tiles, wait for
        neighbour tiles, load them, remaining products and epilogue, integrator bookkeeping,
-       step-end all-gather, whole kernel, publishing the next stage input, unused */
+       step end (error norm, record, all-gather), whole kernel, publishing the next stage input,
+       stage entry */
     double phase_us[6][12];
 } rcmps_stats;
 

@@ -19,8 +19,8 @@
 #include "rcmps_tables.h"
 #include "trajectory.h"
 
-int rc_trajectory_smem_bytes(int nb);
-hipError_t rc_launch_trajectory(int nb, int nblocks, const TrajParams* dparams, hipStream_t stream);
+int rc_trajectory_smem_bytes(int nb, int split);
+hipError_t rc_launch_trajectory(int nb, int split, int nblocks, const TrajParams* dparams, hipStream_t stream);
 
 using rc::cd;
 using rc::Mat;
@@ -891,7 +891,8 @@ static int fg_phi4_impl(rcmps_context* ctx, const rcmps_state* st, double g, con
         ck(hipMemsetAsync(ctx->abort_word.p, 0, 256, ctx->stream));
         ck(hipMemsetAsync(ctx->traj_out.p, 0, RC_NSYS * RC_OUT_STRIDE * 8, ctx->stream));
         ck(hipEventRecord(ctx->ev[0], ctx->stream));
-        ck(rc_launch_trajectory(NB, (int)tasks.size(), static_cast<const TrajParams*>(ctx->traj_params.p),
+        ck(rc_launch_trajectory(NB, (flags & RCMPS_FLAG_SPLIT) ? 1 : 0, (int)tasks.size(),
+                                static_cast<const TrajParams*>(ctx->traj_params.p),
                                 ctx->stream));
         ck(hipEventRecord(ctx->ev[1], ctx->stream));
         std::vector<double> outs(RC_NSYS * RC_OUT_STRIDE);

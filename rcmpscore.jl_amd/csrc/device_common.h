@@ -68,6 +68,10 @@ __device__ __forceinline__ double ld_coh_f64(const double* p) {
 __device__ __forceinline__ void st_coh_f64(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// read-only data written before the launch, read at a wave-uniform index: through the scalar cache
+__device__ __forceinline__ double ld_const(const double* p, int i) {
+    return ((const __attribute__((address_space(4))) double*)p)[__builtin_amdgcn_readfirstlane(i)];
+}
 __device__ __forceinline__ uint32_t ld_flag(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

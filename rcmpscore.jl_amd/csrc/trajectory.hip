@@ -19,6 +19,7 @@
 // deterministic all-gather of per-workgroup records, so every workgroup of a system takes the
 // same controller decision.  Every wait is bounded by a spin limit and watches an abort word.
 #include "device_common.h"
+#include "split.h"
 #include "trajectory.h"
 
 namespace {
@@ -198,9 +199,133 @@ __device__ __forceinline__ void load_row_col_coh(cplx* M, cplx* N, const double*
     }
 }
 
+
+// The task of this workgroup through scalar loads, so that every field is known wave-uniform.
+static_assert(sizeof(TrajTask) % 8 == 0, "task records are copied in 8-byte words");
+__device__ __forceinline__ TrajTask load_task(const TrajTask* tasks, int i) {
+    const __attribute__((address_space(4))) uint64_t* q = (const __attribute__((address_space(4))) uint64_t*)(tasks + i);
+    TrajTask t;
+    uint64_t* w = reinterpret_cast<uint64_t*>(&t);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(TrajTask) / 8); ++k) w[k] = q[k];
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------
+// split precision: LDS layout, panel loads into digit planes, wave-split real products
+// ---------------------------------------------------------------------------------------
+struct LaySplit {
+    static constexpr int P = RC_PANEL_BYTES;
+    static constexpr int XR = 0, XC = P, PB = 2 * P, K0 = 3 * P, K1 = 4 * P, K2 = 5 * P, K3 = 6 * P,
+                         K4 = 7 * P;
+    static constexpr int RED = 8 * P;                  // 4 x 256 doubles
+    static constexpr int SCR = RED + 4 * 256 * 8;      // 64 doubles
+    static constexpr int BYTES = SCR + 64 * 8;
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ double max4(const cplx* v) {
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m = fmax(m, fmax(fabs(v[q].re), fabs(v[q].im)));
+    return m;
+}
+
+// Block maxima of N panels (one value per thread each) -> uniform panel scales S[n].
+template <int N>
+__device__ __forceinline__ void block_scales(const double* m, int* S, double* scr) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+        const double v = wave_max(m[n]);
+        if (lane == 0) scr[8 + 4 * n + wave] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+        const double* q = scr + 8 + 4 * n;
+        S[n] = uni(rc_split_scale(fmax(fmax(q[0], q[1]), fmax(q[2], q[3]))));
+    }
+}
+
+// Thread t holds four consecutive k of one row (row t / 16, k = 4 (t % 16) + q) of a row block,
+// or of one column (column t % 16, k = 4 (t / 16) + q) of a column block; zero beyond Dp.
+template <int NB, bool COH>
+__device__ __forceinline__ void fetch_row4(cplx* v, const double* g, uint32_t gbytes, int b) {
+    constexpr int Dp = 16 * NB;
+    const int t = threadIdx.x, i = t >> 4, k0 = 4 * (t & 15);
+    if (COH) {
+        const __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            v[q] = k0 < Dp ? ld_coh(r, (uint32_t)(((16 * b + i) * Dp + k0 + q) * 16)) : cmake(0, 0);
+    } else {
+        const cplx* gc = reinterpret_cast<const cplx*>(g);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = k0 < Dp ? gc[(16 * b + i) * Dp + k0 + q] : cmake(0, 0);
+    }
+}
+template <int NB, bool COH>
+__device__ __forceinline__ void fetch_col4(cplx* v, const double* g, uint32_t gbytes, int c) {
+    constexpr int Dp = 16 * NB;
+    const int t = threadIdx.x, j = t & 15, k0 = 4 * (t >> 4);
+    if (COH) {
+        const __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            v[q] = k0 < Dp ? ld_coh(r, (uint32_t)(((k0 + q) * Dp + 16 * c + j) * 16)) : cmake(0, 0);
+    } else {
+        const cplx* gc = reinterpret_cast<const cplx*>(g);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = k0 < Dp ? gc[(k0 + q) * Dp + 16 * c + j] : cmake(0, 0);
+    }
+}
+template <int NB>
+__device__ __forceinline__ void store_row4(unsigned char* panel, const cplx* v, int S) {
+    const int t = threadIdx.x, k0 = 4 * (t & 15);
+    if (k0 < 16 * NB) rc_store_digits4(panel, t >> 4, k0, v, S);
+}
+template <int NB>
+__device__ __forceinline__ void store_col4(unsigned char* panel, const cplx* v, int S) {
+    const int t = threadIdx.x, k0 = 4 * (t >> 4);
+    if (k0 < 16 * NB) rc_store_digits4(panel, t & 15, k0, v, S);
+}
+
+// one constant panel (row block b or column block c of a fixed matrix) at kernel start
+template <int NB>
+__device__ __forceinline__ int const_panel(unsigned char* panel, const double* g, bool row, int blk, double* scr) {
+    cplx v[4];
+    if (row) fetch_row4<NB, false>(v, g, 0, blk);
+    else fetch_col4<NB, false>(v, g, 0, blk);
+    const double m = max4(v);
+    int S;
+    block_scales<1>(&m, &S, scr);
+    if (row) store_row4<NB>(panel, v, S);
+    else store_col4<NB>(panel, v, S);
+    __syncthreads();
+    return S;
+}
+
+// Waves 0..3 hold the real products re.re, im.im, re.im, im.re of the same complex tile product;
+// thread t receives element (t/16, t%16) of the complex result.
+__device__ __forceinline__ cplx reduce_split(const double* part, double* red, int wave, int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave * 256 + (4 * (lane >> 4) + q) * 16 + (lane & 15)] = part[q];
+    __syncthreads();
+    const int t = threadIdx.x;
+    const cplx s = cmake(red[t] - red[256 + t], red[512 + t] + red[768 + t]);
+    __syncthreads();
+    return s;
+}
+
 }  // namespace
 
-template <int NB>
+template <int NB, bool SPLIT>
 __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __restrict__ pp) {
     using L = Lay<NB>;
     constexpr int Dp = L::Dp, NT = NB * NB;
@@ -215,15 +340,27 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     cplx* Kp = sm + L::K3;    // R^+ col block (fwd) / R^+ row block (adj)
     cplx* Kc = sm + L::K4;    // C col block (adj channel with E)
     cplx* red = sm + L::RED;
-    double* scratch = reinterpret_cast<double*>(sm + L::NCPLX);
+    // split precision: digit planes of the same operands (FP64 blocks above unused)
+    unsigned char* pXr = smem_raw + LaySplit::XR;
+    unsigned char* pXc = smem_raw + LaySplit::XC;
+    unsigned char* pPb = smem_raw + LaySplit::PB;
+    unsigned char* pK0 = smem_raw + LaySplit::K0;
+    unsigned char* pK1 = smem_raw + LaySplit::K1;
+    unsigned char* pK2 = smem_raw + LaySplit::K2;
+    unsigned char* pK3 = smem_raw + LaySplit::K3;
+    unsigned char* pK4 = smem_raw + LaySplit::K4;
+    double* redd = reinterpret_cast<double*>(smem_raw + LaySplit::RED);
+    double* scratch = SPLIT ? reinterpret_cast<double*>(smem_raw + LaySplit::SCR)
+                            : reinterpret_cast<double*>(sm + L::NCPLX);
     __shared__ int s_abort;
     __shared__ uint32_t* s_addr[64];
     __shared__ double s_tot[16];
 
     const TrajParams& p = *pp;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int spa = (wave == 1 || wave == 3), spb = (wave == 1 || wave == 2);   // this wave's real product
     const int ei = tid >> 4, ej = tid & 15;
-    const TrajTask tk = p.tasks[blockIdx.x];
+    const TrajTask tk = load_task(p.tasks, blockIdx.x);
     const TrajSys S = p.sys[tk.sys];
     const int b = tk.b, c = tk.c, ch = tk.ch, adj = tk.adj;
     const int row = 16 * b + ei, col = 16 * c + ej;
@@ -239,7 +376,24 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     const uint32_t el_off_t = (uint32_t)((col * Dp + row) * 16);   // transposed position
 
     // constant blocks
-    if (!adj) {
+    int sK0 = 0, sK1 = 0, sK2 = 0, sK3 = 0, sK4 = 0;   // split-precision panel scales
+    if constexpr (SPLIT) {
+        // zero every digit plane once: k beyond Dp stays zero
+        for (int i = tid; i < 8 * RC_PANEL_BYTES / 16; i += 256) reinterpret_cast<v4i*>(smem_raw)[i] = v4i{0, 0, 0, 0};
+        __syncthreads();
+        if (!adj) {
+            sK0 = const_panel<NB>(pK0, p.R, true, b, scratch);
+            sK1 = const_panel<NB>(pK1, p.Q, true, b, scratch);
+            sK2 = const_panel<NB>(pK2, p.Qh, false, c, scratch);
+            sK3 = const_panel<NB>(pK3, p.Rh, false, c, scratch);
+        } else {
+            sK0 = const_panel<NB>(pK0, p.R, false, c, scratch);
+            sK1 = const_panel<NB>(pK1, p.Qh, true, b, scratch);
+            sK2 = const_panel<NB>(pK2, p.Q, false, c, scratch);
+            sK3 = const_panel<NB>(pK3, p.Rh, true, b, scratch);
+            if (tk.has_E) sK4 = const_panel<NB>(pK4, p.C, false, c, scratch);
+        }
+    } else if (!adj) {
         load_rowblk<NB, false>(Kl1, p.R, 0, b);
         load_rowblk<NB, false>(Kq, p.Q, 0, b);
         load_colblk<NB, false>(Kx, p.Qh, 0, c);
@@ -264,7 +418,8 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     // one right-hand-side evaluation on the stage input published in slot si with event ev
     // ------------------------------------------------------------------------------------
     // phase timers (constant-rate clock ticks): wait Y, load X, level 1 + publish P, level 2a,
-    // wait P, load P/insertions, level 2b + epilogue, bookkeeping, gather, total, publish Y
+    // wait P, load P/insertions, level 2b + epilogue, integrator bookkeeping, step end (error
+    // norm, record, all-gather), total, publish Y, stage entry
     uint64_t tacc[RC_NTIMERS];
 #pragma unroll
     for (int q = 0; q < RC_NTIMERS; ++q) tacc[q] = 0;
@@ -285,40 +440,77 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             (void)n;
         }
         __syncthreads();
-        lap(7);
+        lap(11);
         if (!wait_flags(ctl, s_addr, 2 * NB, ev, &s_abort)) return false;
         lap(0);
         const double* Ych = S.Y + ((size_t)si * S.nlive + ch) * Dp * Dp * 2;
-        load_row_col_coh<NB>(Xr, Xc, Ych, mat_bytes, b, c);
+        cplx xel = cmake(0, 0);   // own element of the stage input (split path)
+        int sXr = 0, sXc = 0;
+        if constexpr (SPLIT) {
+            cplx vr[4], vc[4];
+            fetch_row4<NB, true>(vr, Ych, mat_bytes, b);
+            fetch_col4<NB, true>(vc, Ych, mat_bytes, c);
+            if (tk.trace_kind == 1) xel = ld_coh(rY, (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes + el_off);
+            const double m[2] = {max4(vr), max4(vc)};
+            int sc[2];
+            block_scales<2>(m, sc, scratch);
+            sXr = sc[0];
+            sXc = sc[1];
+            store_row4<NB>(pXr, vr, sXr);
+            store_col4<NB>(pXc, vc, sXc);
+        } else {
+            load_row_col_coh<NB>(Xr, Xc, Ych, mat_bytes, b, c);
+        }
         __syncthreads();
         lap(1);
         // level 1
         cacc acc;
-        cacc_zero(acc);
-        if (!adj) tile_product<NB>(acc, Kl1, Xc, wave, lane);
-        else tile_product<NB>(acc, Xr, Kl1, wave, lane);
-        cplx Pe = reduce_tile(acc, red, wave, lane);
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        cplx Pe;
+        if constexpr (SPLIT) {
+            if (!adj) rc_split_product(part, pK0, spa, pXc, spb, -(sK0 + sXc), lane);
+            else rc_split_product(part, pXr, spa, pK0, spb, -(sXr + sK0), lane);
+            Pe = reduce_split(part, redd, wave, lane);
+        } else {
+            cacc_zero(acc);
+            if (!adj) tile_product<NB>(acc, Kl1, Xc, wave, lane);
+            else tile_product<NB>(acc, Xr, Kl1, wave, lane);
+            Pe = reduce_tile(acc, red, wave, lane);
+        }
         const uint32_t poff = (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes;
         st_coh(rP, poff + el_off, Pe);
         if (tk.has_E) {
-            cacc_zero(acc);
-            tile_product<NB>(acc, Xr, Kc, wave, lane);
-            cplx Ee = reduce_tile(acc, red, wave, lane);
+            cplx Ee;
+            if constexpr (SPLIT) {
+                double pe[4] = {0.0, 0.0, 0.0, 0.0};
+                rc_split_product(pe, pXr, spa, pK4, spb, -(sXr + sK4), lane);
+                Ee = reduce_split(pe, redd, wave, lane);
+            } else {
+                cacc_zero(acc);
+                tile_product<NB>(acc, Xr, Kc, wave, lane);
+                Ee = reduce_tile(acc, red, wave, lane);
+            }
             st_coh(rE, (uint32_t)si * mat_bytes + el_off, Ee);
         }
         double trv = 0.0;
         if (tk.trace_kind == 2 && b == c && ei == ej && real_el) trv = Pe.re;
         if (tk.trace_kind == 1 && real_el) {
-            cplx x = Xr[ei * L::RS + col];
+            const cplx x = SPLIT ? xel : Xr[ei * L::RS + col];
             trv = x.re * trop_e.re + x.im * trop_e.im;   // Re(x conj(o))
         }
         publish(flagP + my_tile, ev);
         if (tk.has_E) publish(S.flagE + my_tile, ev);
         lap(2);
         // level 2a
-        cacc_zero(acc);
-        tile_product<NB>(acc, Kq, Xc, wave, lane);
-        tile_product<NB>(acc, Xr, Kx, wave, lane);
+        if constexpr (SPLIT) {
+            part[0] = part[1] = part[2] = part[3] = 0.0;
+            rc_split_product(part, pK1, spa, pXc, spb, -(sK1 + sXc), lane);
+            rc_split_product(part, pXr, spa, pK2, spb, -(sXr + sK2), lane);
+        } else {
+            cacc_zero(acc);
+            tile_product<NB>(acc, Kq, Xc, wave, lane);
+            tile_product<NB>(acc, Xr, Kx, wave, lane);
+        }
         __syncthreads();
         lap(3);
         // wait for the P tiles of this channel and the insertion source tiles
@@ -335,8 +527,14 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         if (!wait_flags(ctl, s_addr, nw, ev, &s_abort)) return false;
         lap(4);
         const double* Pch = S.P + ((size_t)si * S.nlive + ch) * Dp * Dp * 2;
-        if (!adj) load_rowblk<NB, true>(Pb, Pch, mat_bytes, b);
-        else load_colblk<NB, true>(Pb, Pch, mat_bytes, c);
+        cplx vp[4];
+        if constexpr (SPLIT) {
+            if (!adj) fetch_row4<NB, true>(vp, Pch, mat_bytes, b);
+            else fetch_col4<NB, true>(vp, Pch, mat_bytes, c);
+        } else {
+            if (!adj) load_rowblk<NB, true>(Pb, Pch, mat_bytes, b);
+            else load_colblk<NB, true>(Pb, Pch, mat_bytes, c);
+        }
         cplx ins = cmake(0, 0);
         if (tk.edge_src >= 0) {
             cplx s1, s2;
@@ -350,11 +548,25 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             }
             ins = cscale(tk.edge_coef, cadd(s1, cconj(s2)));
         }
+        int sPb = 0;
+        if constexpr (SPLIT) {
+            const double m = max4(vp);
+            block_scales<1>(&m, &sPb, scratch);
+            if (!adj) store_row4<NB>(pPb, vp, sPb);
+            else store_col4<NB>(pPb, vp, sPb);
+        }
         __syncthreads();
         lap(5);
-        if (!adj) tile_product<NB>(acc, Pb, Kp, wave, lane);
-        else tile_product<NB>(acc, Kp, Pb, wave, lane);
-        cplx Le = reduce_tile(acc, red, wave, lane);
+        cplx Le;
+        if constexpr (SPLIT) {
+            if (!adj) rc_split_product(part, pPb, spa, pK3, spb, -(sPb + sK3), lane);
+            else rc_split_product(part, pK3, spa, pPb, spb, -(sK3 + sPb), lane);
+            Le = reduce_split(part, redd, wave, lane);
+        } else {
+            if (!adj) tile_product<NB>(acc, Pb, Kp, wave, lane);
+            else tile_product<NB>(acc, Kp, Pb, wave, lane);
+            Le = reduce_tile(acc, red, wave, lane);
+        }
         cplx k = cadd(cscale(jac, Le), cscale(src, cadd(ins, fixed_e)));
         if (!real_el) k = cmake(0, 0);
         k_out = k;
@@ -391,7 +603,9 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
 
     // row j of RC_COEF: accA coefficients in the frame of stage j, then B, E5, E3, D[0..3]
     __shared__ double s_coef[RC_NSTAGES_EXT][RC_COEF_ROW];
+    __shared__ double s_cnode[RC_NSTAGES_EXT];
     for (int i = tid; i < RC_NSTAGES_EXT * RC_COEF_ROW; i += 256) (&s_coef[0][0])[i] = (&RC_COEF[0][0])[i];
+    if (tid < RC_NSTAGES_EXT) s_cnode[tid] = RC_C[tid];
     __syncthreads();
     auto absorb = [&](int j_in, cplx kj) {
         const double* c = &s_coef[uni(j_in)][0];
@@ -453,6 +667,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         } else if (slot == RC_NSTAGES - 1) {
             absorb(slot, kn);
             shift_to(slot);
+            lap(7);
             ynew = cadd(y, cscale(h, accB));
             double e5p = 0.0, e3p = 0.0;
             if (real_el) {
@@ -471,19 +686,31 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
                 else if (tid < 2 + RC_NSTAGES) v = s_trp[tid - 2];
                 st_coh_f64(S.rec + ((size_t)par * S.ntasks + my_rank) * RC_REC + tid, v);
             }
-            lap(7);
+            lap(8);
             publish(S.flagR + my_rank, (uint32_t)attempts);
             if (tid < 64) s_addr[lane] = S.flagR + (lane < S.ntasks ? lane : 0);
             __syncthreads();
             if (!(alive = wait_flags(ctl, s_addr, S.ntasks, (uint32_t)attempts, &s_abort))) break;
             if (tid < 64) {
+                // every record in flight at once, then one deterministic reduction per entry
+                const __amdgpu_buffer_rsrc_t rR = rc_rsrc(S.rec, (uint32_t)(2 * S.ntasks * RC_REC * 8));
+                const uint32_t base = (uint32_t)(((size_t)par * S.ntasks + lane) * RC_REC * 8);
+                cplx v[(2 + RC_NSTAGES) / 2];
 #pragma unroll
-                for (int q = 0; q < 2 + RC_NSTAGES; ++q) {
-                    double v = 0.0;
-                    if (lane < S.ntasks) v = ld_coh_f64(S.rec + ((size_t)par * S.ntasks + lane) * RC_REC + q);
+                for (int q = 0; q < (2 + RC_NSTAGES) / 2; ++q)
+                    v[q] = lane < S.ntasks ? ld_coh(rR, base + 16 * q) : cmake(0, 0);
 #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                    if (lane == 0) s_tot[q] = v;
+                for (int q = 0; q < (2 + RC_NSTAGES) / 2; ++q) {
+                    double x = v[q].re, z = v[q].im;
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) {
+                        x += __shfl_xor(x, off);
+                        z += __shfl_xor(z, off);
+                    }
+                    if (lane == 0) {
+                        s_tot[2 * q] = x;
+                        s_tot[2 * q + 1] = z;
+                    }
                 }
             }
             __syncthreads();
@@ -523,7 +750,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             k12 = kn;
             absorb(slot, kn);
             shift_to(slot);
-            if (next_point < p.npoints && p.points[next_point] <= t_new) {
+            if (next_point < p.npoints && ld_const(p.points, next_point) <= t_new) {
                 ++dense;
                 next = RC_NSTAGES + 1;
             } else {
@@ -541,8 +768,8 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             cplx dd[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) dd[r] = cscale(h, accD[r]);
-            while (next_point < p.npoints && p.points[next_point] <= t_new) {
-                const double pt = p.points[next_point];
+            while (next_point < p.npoints && ld_const(p.points, next_point) <= t_new) {
+                const double pt = ld_const(p.points, next_point);
                 cplx v;
                 if (pt == t_new) {
                     v = ynew;
@@ -583,7 +810,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
             ++attempts;
             reset_acc();
             // source values at every stage time of this attempt (stages 1..15), one lane each
-            if (tid >= 1 && tid < RC_NSTAGES_EXT) s_src[tid] = rc_source(t + RC_C[tid] * h, s_cheb);
+            if (tid >= 1 && tid < RC_NSTAGES_EXT) s_src[tid] = rc_source(t + s_cnode[tid] * h, s_cheb);
             __syncthreads();
             next = 1;
         }
@@ -614,7 +841,6 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         S.out[RC_OUT_NEXTPOINT] = next_point;
         lap(7);
         tacc[9] = __builtin_amdgcn_s_memrealtime() - tstart;
-        tacc[11] = 0;
 #pragma unroll
         for (int q = 0; q < RC_NTIMERS; ++q) S.out[RC_OUT_TIMERS + q] = (double)tacc[q];
     }
@@ -623,25 +849,30 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
 // ---------------------------------------------------------------------------------------
 // host launcher
 // ---------------------------------------------------------------------------------------
-int rc_trajectory_smem_bytes(int nb) {
+int rc_trajectory_smem_bytes(int nb, int split) {
+    if (nb < 1 || nb > 4) return -1;
+    if (split) return LaySplit::BYTES;
     switch (nb) {
         case 1: return Lay<1>::BYTES;
         case 2: return Lay<2>::BYTES;
         case 3: return Lay<3>::BYTES;
-        case 4: return Lay<4>::BYTES;
-        default: return -1;
+        default: return Lay<4>::BYTES;
     }
 }
 
-hipError_t rc_launch_trajectory(int nb, int nblocks, const TrajParams* dparams, hipStream_t stream) {
+hipError_t rc_launch_trajectory(int nb, int split, int nblocks, const TrajParams* dparams, hipStream_t stream) {
     void* args[] = {(void*)&dparams};
     const void* fn = nullptr;
-    int bytes = rc_trajectory_smem_bytes(nb);
-    switch (nb) {
-        case 1: fn = (const void*)rc_trajectory<1>; break;
-        case 2: fn = (const void*)rc_trajectory<2>; break;
-        case 3: fn = (const void*)rc_trajectory<3>; break;
-        case 4: fn = (const void*)rc_trajectory<4>; break;
+    int bytes = rc_trajectory_smem_bytes(nb, split);
+    switch (nb * 2 + (split ? 1 : 0)) {
+        case 2: fn = (const void*)rc_trajectory<1, false>; break;
+        case 3: fn = (const void*)rc_trajectory<1, true>; break;
+        case 4: fn = (const void*)rc_trajectory<2, false>; break;
+        case 5: fn = (const void*)rc_trajectory<2, true>; break;
+        case 6: fn = (const void*)rc_trajectory<3, false>; break;
+        case 7: fn = (const void*)rc_trajectory<3, true>; break;
+        case 8: fn = (const void*)rc_trajectory<4, false>; break;
+        case 9: fn = (const void*)rc_trajectory<4, true>; break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

@@ -24,6 +24,7 @@ STATUS = {
 SYSTEMS = ("kinetic forward", "kinetic adjoint", "kinetic-commutator forward",
            "kinetic-commutator adjoint", "phi4 forward", "phi4 adjoint")
 FLAG_NO_BOUNDARY = 1
+FLAG_SPLIT = 2
 
 
 class SolverFailure(RuntimeError):

@@ -16,6 +16,13 @@ from . import _lib
 from .state import IntegratorParameters, State
 
 PARALLEL = (None, "serial", "terms", "trajectories", "both")
+PRECISIONS = ("fp64", "split")
+
+
+def _flags(boundary: bool, precision: str) -> int:
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}")
+    return (0 if boundary else _lib.FLAG_NO_BOUNDARY) | (_lib.FLAG_SPLIT if precision == "split" else 0)
 
 
 def _integrator(p: IntegratorParameters, initial_step: float = 0.01) -> _lib.Integrator:
@@ -26,8 +33,11 @@ def _integrator(p: IntegratorParameters, initial_step: float = 0.01) -> _lib.Int
 
 
 def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: IntegratorParameters | None = None,
-                metric_reg: float | None = None):
-    """(energy, gradient, raw covector, raw environment, stats dict) on device tensors."""
+                metric_reg: float | None = None, precision: str = "fp64"):
+    """(energy, gradient, raw covector, raw environment, stats dict) on device tensors.
+
+    precision="split" runs the trajectory products on the INT8 MFMA over exact 8-bit slices of
+    the FP64 operands; everything else stays FP64."""
     lib = _lib.load()
     ctx = _lib.context(s.device)
     ip = _integrator(integrator or s.integrator_param)
@@ -41,7 +51,7 @@ def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: Integr
     env = torch.empty_like(G)
     e = ctypes.c_double()
     stats = _lib.Stats()
-    flags = 0 if boundary else _lib.FLAG_NO_BOUNDARY
+    flags = _flags(boundary, precision)
     st = lib.rcmps_fg_phi4(ctx, s.handle, float(g), ctypes.byref(ip), float(reg), flags, ctypes.byref(e),
                            G.data_ptr(), cov.data_ptr(), env.data_ptr(), ctypes.byref(stats))
     _lib.check(st, "fg_phi4", stats)
@@ -49,7 +59,7 @@ def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: Integr
 
 
 def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str = "ode", parallel=None,
-            boundary: bool = True, return_stats: bool = False):
+            boundary: bool = True, return_stats: bool = False, precision: str = "fp64"):
     """fg_phi4(s, g) -> (energy, gradient) with dE[W] = metric(s, gradient, W) for horizontal W.
 
     gradient=False returns the energy only.  `parallel` is accepted for signature compatibility:
@@ -58,7 +68,7 @@ def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str =
         raise ValueError("parallel must be :serial, :terms, :trajectories, or :both")
     if kinetic_backend != "ode":
         raise NotImplementedError("only the finite-window ODE kinetic backend is built")
-    energy, G, _, _, stats = fg_phi4_raw(s, g, boundary=boundary)
+    energy, G, _, _, stats = fg_phi4_raw(s, g, boundary=boundary, precision=precision)
     if not gradient:
         return (energy, stats) if return_stats else energy
     grad = G.cpu().numpy()
@@ -66,7 +76,7 @@ def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str =
 
 
 def fg_phi4_rk(s: State, g: float, *, gradient: bool = True, boundary: bool = True,
-               integrator: IntegratorParameters | None = None):
+               integrator: IntegratorParameters | None = None, precision: str = "fp64"):
     """(energy, grad_R, grad_K, stats): the energy of State(R, K) and its Euclidean gradient in
     (R, K) as device tensors, dE = Re sum(conj(grad_R) dR + conj(grad_K) dK) for Hermitian dK.
     gradient=False integrates the forward trajectories only and returns None gradients."""
@@ -80,7 +90,7 @@ def fg_phi4_rk(s: State, g: float, *, gradient: bool = True, boundary: bool = Tr
         GK = torch.empty_like(GR)
     e = ctypes.c_double()
     stats = _lib.Stats()
-    flags = 0 if boundary else _lib.FLAG_NO_BOUNDARY
+    flags = _flags(boundary, precision)
     st = lib.rcmps_fg_phi4_rk(ctx, s.handle, float(g), ctypes.byref(ip), flags, ctypes.byref(e),
                               GR.data_ptr() if gradient else None, GK.data_ptr() if gradient else None,
                               ctypes.byref(stats))

@@ -45,3 +45,10 @@ def test_status_strings_without_device():
 def test_package_import_does_not_touch_the_device():
     import rcmps_gpu
     assert callable(rcmps_gpu.fg_phi4)
+
+
+def test_flag_values_match_header():
+    import rcmps_gpu._lib as L
+    flags = dict(re.findall(r"#define RCMPS_FLAG_([A-Z_]+)\s+(\d+)", HEADER.read_text()))
+    assert flags == {"NO_BOUNDARY": "1", "SPLIT": "2"}
+    assert (L.FLAG_NO_BOUNDARY, L.FLAG_SPLIT) == (1, 2)
