@@ -95,6 +95,17 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
                   double metric_reg, int flags, double* energy, double* grad, double* raw_cov,
                   double* raw_env, rcmps_stats* stats);
 
+/* fg_phi4 over n independent states (any mix of bond dimensions and couplings g[i]) with one
+   integrator: energies[i] (host) and, where grads and grads[i] are non-null, the Riemannian
+   gradient into the device buffer grads[i] (D_i x D_i).  States of the same padded size share one
+   persistent launch in which their trajectories are gang-scheduled in rounds of resident
+   workgroups.  stats may be null or an array of n.  Returns the first failure (each stats[i]
+   holds its own status); the evaluations that succeeded are complete.  Replaces a loop of
+   fg_phi4(s_i, g_i) calls (src/observables.jl:257); the reference has no batched form. */
+int rcmps_fg_phi4_batch(rcmps_context* ctx, int n, const rcmps_state* const* states, const double* g,
+                        const rcmps_integrator* ip, double metric_reg, int flags, double* energies,
+                        double* const* grads, rcmps_stats* stats);
+
 /* Energy and its Euclidean gradient in (R, K), for Q = -iK - R^+R/2 with rho0 re-solved:
    dE = Re sum_ij [conj(grad_R_ij) dR_ij + conj(grad_K_ij) dK_ij] for every dR and every
    Hermitian dK (grad_K is Hermitian), i.e. torch's convention for a real function of complex

@@ -10,6 +10,7 @@
 #include <complex>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,7 @@
 #include "trajectory.h"
 
 int rc_trajectory_smem_bytes(int nb, int split);
+hipError_t rc_trajectory_capacity(int nb, int split, int* blocks);
 hipError_t rc_launch_trajectory(int nb, int split, int nblocks, const TrajParams* dparams, hipStream_t stream);
 
 using rc::cd;
@@ -49,6 +51,21 @@ struct DevBuf {
 
 struct Fail {
     int code;
+};
+
+constexpr int kLive[RC_NSYS] = {1, 2, 1, 2, 3, 4};     // live channels per system
+constexpr int kHasScalar[RC_NSYS] = {1, 0, 1, 0, 1, 0};
+
+// trajectory workspace of one evaluation (one state) of a launch
+struct TrajBufs {
+    DevBuf Y[RC_NSYS], P[RC_NSYS], rec[RC_NSYS], samples[RC_NSYS], E, fixed, flags, out;
+    size_t flag_bytes = 0;
+    void release() {
+        for (int s = 0; s < RC_NSYS; ++s) {
+            Y[s].release(); P[s].release(); rec[s].release(); samples[s].release();
+        }
+        E.release(); fixed.release(); flags.release(); out.release();
+    }
 };
 
 inline void ck(hipError_t e) {
@@ -118,9 +135,9 @@ inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 struct rcmps_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    // trajectory workspaces
-    DevBuf traj_Y[RC_NSYS], traj_P[RC_NSYS], traj_E, traj_rec[RC_NSYS], traj_flags, traj_samples[RC_NSYS],
-        traj_out, traj_params, traj_tasks, traj_points, traj_fixed, abort_word;
+    // trajectory workspaces: per evaluation of a launch, and the launch description
+    std::vector<std::unique_ptr<TrajBufs>> tb;
+    DevBuf traj_params, traj_tasks, traj_sys, traj_sched, traj_points, abort_word;
     // pullback / misc workspaces
     DevBuf nodes_w, nodes_j, tmpM, partial, scratch, small;
     // krylov workspaces
@@ -131,7 +148,6 @@ struct rcmps_context {
     int q_n = 0;
     double q_max_point = -1.0;
     std::vector<double> q_pts;
-    int nlive[RC_NSYS] = {1, 2, 1, 2, 3, 4};
     hipEvent_t ev[8];
     double clock_mhz = 100.0;   // constant-rate device clock
 };
@@ -527,18 +543,13 @@ void rcmps_context_destroy(rcmps_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* all[] = {&ctx->traj_E, &ctx->traj_flags, &ctx->traj_out, &ctx->traj_params, &ctx->traj_tasks,
-                     &ctx->traj_points, &ctx->traj_fixed, &ctx->abort_word, &ctx->nodes_w, &ctx->nodes_j,
+    DevBuf* all[] = {&ctx->traj_params, &ctx->traj_tasks, &ctx->traj_sys, &ctx->traj_sched,
+                     &ctx->traj_points, &ctx->abort_word, &ctx->nodes_w, &ctx->nodes_j,
                      &ctx->tmpM, &ctx->partial, &ctx->scratch, &ctx->small, &ctx->kry_V, &ctx->kry_w,
                      &ctx->kry_t1, &ctx->kry_t2, &ctx->kry_x, &ctx->kry_b, &ctx->kry_z, &ctx->kry_h,
                      &ctx->kry_tmp};
     for (auto* b : all) b->release();
-    for (int s = 0; s < RC_NSYS; ++s) {
-        ctx->traj_Y[s].release();
-        ctx->traj_P[s].release();
-        ctx->traj_rec[s].release();
-        ctx->traj_samples[s].release();
-    }
+    for (auto& b : ctx->tb) b->release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -737,374 +748,495 @@ int rcmps_boundary_environment(rcmps_context* ctx, const rcmps_state* st, const 
     });
 }
 
-static int fg_phi4_impl(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip,
-                        double metric_reg, int flags, double* energy, double* grad, double* raw_cov,
-                        double* raw_env, double* grad_R, double* grad_K, rcmps_stats* stats) {
-    if (!ctx || !st || !ip || !energy) return RCMPS_ERR_ARGUMENT;
-    if (ip->n_quad < 8 || !(ip->max_point > 0) || !(ip->atol > 0) || !(ip->rtol > 0) || ip->maxiters < 1 ||
-        !(metric_reg >= 0) || !std::isfinite(g))
-        return RCMPS_ERR_ARGUMENT;
-    if (std::sqrt(ip->max_point) >= RC_SOURCE_TMAX) return RCMPS_ERR_UNSUPPORTED;
-    const int D = st->D, Dp = st->Dp, NB = Dp / 16;
-    if (NB > 4) return RCMPS_ERR_UNSUPPORTED;
-    rcmps_stats local{};
-    rcmps_stats* S = stats ? stats : &local;
-    std::memset(S, 0, sizeof(*S));
-    auto t_start = std::chrono::steady_clock::now();
-    int rc = guard([&] {
-        ck(hipSetDevice(ctx->device));
-        Dev dv{ctx, Dp, D};
-        const size_t mb = (size_t)Dp * Dp * 16;
-        const size_t mcount = (size_t)Dp * Dp * 2;   // doubles per matrix
-        const int nn = 2 * ip->n_quad, npts = nn + 1;
-        if (ctx->q_n != ip->n_quad || ctx->q_max_point != ip->max_point) {
-            std::vector<double> pts, wts;
-            quadrature(ip->n_quad, ip->max_point, pts, wts);
-            pts.push_back(std::sqrt(ip->max_point));
-            std::vector<double> wv(nn), jv(nn);
-            for (int i = 0; i < nn; ++i) {
-                wv[i] = wts[i] * 2 * std::fabs(pts[i]);
-                jv[i] = wts[i] * host_source(pts[i]);
-            }
-            ck(ctx->traj_points.ensure((size_t)npts * 8));
-            ck(ctx->nodes_w.ensure((size_t)nn * 8));
-            ck(ctx->nodes_j.ensure((size_t)nn * 8));
-            ck(hipMemcpyAsync(ctx->traj_points.d(), pts.data(), (size_t)npts * 8, hipMemcpyHostToDevice, ctx->stream));
-            ck(hipMemcpyAsync(ctx->nodes_w.d(), wv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
-            ck(hipMemcpyAsync(ctx->nodes_j.d(), jv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
-            ck(hipStreamSynchronize(ctx->stream));
-            ctx->q_n = ip->n_quad;
-            ctx->q_max_point = ip->max_point;
-            ctx->q_pts = pts;
+namespace {
+
+// One fg_phi4 evaluation of a batch: its state, coupling and outputs.
+struct EvalReq {
+    const rcmps_state* st = nullptr;
+    double g = 0.0;
+    double* energy = nullptr;                                    // host
+    double *grad = nullptr, *raw_cov = nullptr, *raw_env = nullptr, *grad_R = nullptr, *grad_K = nullptr;
+    rcmps_stats* S = nullptr;
+    bool adjoint() const { return grad || raw_cov || raw_env || grad_R || grad_K; }
+};
+
+
+void ensure_quadrature(rcmps_context* ctx, const rcmps_integrator* ip) {
+    if (ctx->q_n == ip->n_quad && ctx->q_max_point == ip->max_point) return;
+    const int nn = 2 * ip->n_quad, npts = nn + 1;
+    std::vector<double> pts, wts;
+    quadrature(ip->n_quad, ip->max_point, pts, wts);
+    pts.push_back(std::sqrt(ip->max_point));
+    std::vector<double> wv(nn), jv(nn);
+    for (int i = 0; i < nn; ++i) {
+        wv[i] = wts[i] * 2 * std::fabs(pts[i]);
+        jv[i] = wts[i] * host_source(pts[i]);
+    }
+    ck(ctx->traj_points.ensure((size_t)npts * 8));
+    ck(ctx->nodes_w.ensure((size_t)nn * 8));
+    ck(ctx->nodes_j.ensure((size_t)nn * 8));
+    ck(hipMemcpyAsync(ctx->traj_points.d(), pts.data(), (size_t)npts * 8, hipMemcpyHostToDevice, ctx->stream));
+    ck(hipMemcpyAsync(ctx->nodes_w.d(), wv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
+    ck(hipMemcpyAsync(ctx->nodes_j.d(), jv.data(), (size_t)nn * 8, hipMemcpyHostToDevice, ctx->stream));
+    ck(hipStreamSynchronize(ctx->stream));
+    ctx->q_n = ip->n_quad;
+    ctx->q_max_point = ip->max_point;
+    ctx->q_pts = pts;
+}
+
+// Fixed insertion sources, the six systems and their tasks for one state (src/observables.jl:7-29:
+// kinetic edges sqrt2 with op R or [Q,R], :phi^4: edges k -> k+1 with coefficient k).
+void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool adjoint, int npts,
+                 std::vector<TrajSys>& sys, std::vector<TrajTask>& tasks) {
+    const int D = st->D, Dp = st->Dp, NB = Dp / 16, NT = NB * NB;
+    Dev dv{ctx, Dp, D};
+    const size_t mb = (size_t)Dp * Dp * 16;
+    const size_t mcount = (size_t)Dp * Dp * 2;
+    ck(tb.fixed.ensure(6 * mb));
+    double* fx = tb.fixed.d();
+    double* fK1F = fx;
+    double* fK1A = fx + mcount;
+    double* fK2F = fx + 2 * mcount;
+    double* fK2A = fx + 3 * mcount;
+    double* fQF = fx + 4 * mcount;
+    double* fQA = fx + 5 * mcount;
+    const double s2 = std::sqrt(2.0);
+    dv.gemm(0, st->R.d(), 0, st->rho.d(), fK1F, s2);                 // sqrt2 R rho
+    dv.axpby(cd(s2, 0), st->Rh.d(), cd(0, 0), nullptr, fK1A);         // sqrt2 R^+
+    dv.gemm(0, st->C.d(), 0, st->rho.d(), fK2F, s2);                 // sqrt2 C rho
+    dv.axpby(cd(s2, 0), st->Ch.d(), cd(0, 0), nullptr, fK2A);         // sqrt2 C^+
+    dv.gemm(0, st->R.d(), 0, st->rho.d(), fQF);                      // R rho + rho R^+
+    dv.gemm(0, st->rho.d(), 0, st->Rh.d(), fQF, 1.0, 1.0);
+    dv.axpby(cd(4, 0), st->R.d(), cd(4, 0), st->Rh.d(), fQA);         // 4 (R + R^+)
+
+    size_t flag_words = 0;
+    for (int s = 0; s < RC_NSYS; ++s) flag_words += (size_t)2 * kLive[s] * NT + NT + 64;
+    ck(tb.flags.ensure(flag_words * 4));
+    tb.flag_bytes = flag_words * 4;
+    ck(tb.out.ensure(RC_NSYS * RC_OUT_STRIDE * 8));
+    ck(tb.E.ensure((size_t)RC_NSTAGE_BUF * mb));
+    uint32_t* fl = static_cast<uint32_t*>(tb.flags.p);
+    const int sys_base = (int)sys.size();
+    for (int s = 0; s < RC_NSYS; ++s) {
+        TrajSys Sy{};
+        Sy.nlive = kLive[s];
+        Sy.has_scalar = kHasScalar[s];
+        Sy.adj = s % 2;
+        Sy.first_task = (int)tasks.size();
+        Sy.ntasks = !adjoint && s % 2 ? 0 : kLive[s] * NT;
+        Sy.D = D;
+        Sy.n_elems = (double)kLive[s] * D * D + kHasScalar[s];
+        Sy.Q = st->Q.d(); Sy.Qh = st->Qh.d(); Sy.R = st->R.d(); Sy.Rh = st->Rh.d(); Sy.C = st->C.d();
+        Sy.flagY = fl; fl += kLive[s] * NT;
+        Sy.flagP = fl; fl += kLive[s] * NT;
+        Sy.flagE = fl; fl += NT;
+        Sy.flagR = fl; fl += 64;
+        Sy.out = tb.out.d() + s * RC_OUT_STRIDE;
+        if (Sy.ntasks > 64) throw Fail{RCMPS_ERR_UNSUPPORTED};
+        if (Sy.ntasks > 0) {
+            if ((size_t)npts * kLive[s] * mb >= (size_t)1 << 31) throw Fail{RCMPS_ERR_UNSUPPORTED};
+            ck(tb.Y[s].ensure((size_t)RC_NSTAGE_BUF * kLive[s] * mb));
+            ck(tb.P[s].ensure((size_t)RC_NSTAGE_BUF * kLive[s] * mb));
+            ck(tb.rec[s].ensure((size_t)2 * Sy.ntasks * RC_REC * 8));
+            ck(tb.samples[s].ensure((size_t)npts * kLive[s] * mb));
+            Sy.Y = tb.Y[s].d();
+            Sy.P = tb.P[s].d();
+            Sy.E = s == 3 ? tb.E.d() : nullptr;
+            Sy.rec = tb.rec[s].d();
+            Sy.samples = tb.samples[s].d();
         }
-
-        // fixed insertion sources (6 matrices)
-        ck(ctx->traj_fixed.ensure(6 * mb));
-        double* fx = ctx->traj_fixed.d();
-        double* fK1F = fx;
-        double* fK1A = fx + mcount;
-        double* fK2F = fx + 2 * mcount;
-        double* fK2A = fx + 3 * mcount;
-        double* fQF = fx + 4 * mcount;
-        double* fQA = fx + 5 * mcount;
-        const double s2 = std::sqrt(2.0);
-        dv.gemm(0, st->R.d(), 0, st->rho.d(), fK1F, s2);                 // sqrt2 R rho
-        dv.axpby(cd(s2, 0), st->Rh.d(), cd(0, 0), nullptr, fK1A);         // sqrt2 R^+
-        dv.gemm(0, st->C.d(), 0, st->rho.d(), fK2F, s2);                 // sqrt2 C rho
-        dv.axpby(cd(s2, 0), st->Ch.d(), cd(0, 0), nullptr, fK2A);         // sqrt2 C^+
-        dv.gemm(0, st->R.d(), 0, st->rho.d(), fQF);                      // R rho + rho R^+
-        dv.gemm(0, st->rho.d(), 0, st->Rh.d(), fQF, 1.0, 1.0);
-        dv.axpby(cd(4, 0), st->R.d(), cd(4, 0), st->Rh.d(), fQA);         // 4 (R + R^+)
-
-        // systems and tasks; the energy alone needs only the forward trajectories
-        const bool energy_only = !grad && !raw_cov && !raw_env && !grad_R && !grad_K;
-        const int NT = NB * NB;
-        const int nlive[RC_NSYS] = {1, 2, 1, 2, 3, 4};
-        const int has_scalar[RC_NSYS] = {1, 0, 1, 0, 1, 0};
-        TrajParams P{};
-        std::vector<TrajTask> tasks;
-        size_t flag_words = 0;
-        for (int s = 0; s < RC_NSYS; ++s) flag_words += (size_t)2 * nlive[s] * NT + NT + 64;
-        ck(ctx->traj_flags.ensure(flag_words * 4));
-        ck(ctx->traj_out.ensure(RC_NSYS * RC_OUT_STRIDE * 8));
-        ck(ctx->traj_E.ensure((size_t)RC_NSTAGE_BUF * mb));
-        uint32_t* fl = static_cast<uint32_t*>(ctx->traj_flags.p);
-        for (int s = 0; s < RC_NSYS; ++s) {
-            TrajSys& Sy = P.sys[s];
-            Sy.nlive = nlive[s];
-            Sy.has_scalar = has_scalar[s];
-            Sy.adj = s % 2;
-            Sy.first_task = (int)tasks.size();
-            Sy.ntasks = energy_only && s % 2 ? 0 : nlive[s] * NT;
-            if (Sy.ntasks == 0) continue;
-            Sy.n_elems = (double)nlive[s] * D * D + has_scalar[s];
-            ck(ctx->traj_Y[s].ensure((size_t)RC_NSTAGE_BUF * nlive[s] * mb));
-            ck(ctx->traj_P[s].ensure((size_t)RC_NSTAGE_BUF * nlive[s] * mb));
-            ck(ctx->traj_rec[s].ensure((size_t)2 * Sy.ntasks * RC_REC * 8));
-            ck(ctx->traj_samples[s].ensure((size_t)npts * nlive[s] * mb));
-            if ((size_t)npts * nlive[s] * mb >= (size_t)1 << 31) throw Fail{RCMPS_ERR_UNSUPPORTED};
-            Sy.Y = ctx->traj_Y[s].d();
-            Sy.P = ctx->traj_P[s].d();
-            Sy.E = s == 3 ? ctx->traj_E.d() : nullptr;
-            Sy.rec = ctx->traj_rec[s].d();
-            Sy.flagY = fl; fl += nlive[s] * NT;
-            Sy.flagP = fl; fl += nlive[s] * NT;
-            Sy.flagE = fl; fl += NT;
-            Sy.flagR = fl; fl += 64;
-            Sy.samples = ctx->traj_samples[s].d();
-            Sy.out = ctx->traj_out.d() + s * RC_OUT_STRIDE;
-            const int term = s / 2, adj = s % 2;
-            for (int ch = 0; ch < nlive[s]; ++ch)
-                for (int b = 0; b < NB; ++b)
-                    for (int c = 0; c < NB; ++c) {
-                        TrajTask t{};
-                        t.sys = s; t.ch = ch; t.b = b; t.c = c; t.adj = adj;
-                        t.edge_src = -1;
-                        if (term < 2) {
-                            const double* O = term == 0 ? st->R.d() : st->C.d();
-                            if (!adj) {
-                                t.fixed = term == 0 ? fK1F : fK2F;
-                                t.trace_kind = 1;
-                                t.trace_op = O;
-                                t.trace_coef = 2 * s2;
-                            } else if (ch == 0) {
-                                t.edge_src = 1;
-                                t.edge_from_E = term == 1;
-                                t.edge_coef = s2;
-                            } else {
-                                t.fixed = term == 0 ? fK1A : fK2A;
-                                t.has_E = term == 1;
-                            }
-                        } else if (!adj) {
-                            if (ch == 0) t.fixed = fQF;
-                            else { t.edge_src = ch - 1; t.edge_coef = ch + 1; }
-                            if (ch == 2) { t.trace_kind = 2; t.trace_coef = 8.0; }
+        const int term = s / 2, adj = s % 2;
+        for (int ch = 0; ch < kLive[s] && Sy.ntasks > 0; ++ch)
+            for (int b = 0; b < NB; ++b)
+                for (int c = 0; c < NB; ++c) {
+                    TrajTask t{};
+                    t.sys = sys_base + s; t.ch = ch; t.b = b; t.c = c; t.adj = adj;
+                    t.edge_src = -1;
+                    if (term < 2) {
+                        const double* O = term == 0 ? st->R.d() : st->C.d();
+                        if (!adj) {
+                            t.fixed = term == 0 ? fK1F : fK2F;
+                            t.trace_kind = 1;
+                            t.trace_op = O;
+                            t.trace_coef = 2 * s2;
+                        } else if (ch == 0) {
+                            t.edge_src = 1;
+                            t.edge_from_E = term == 1;
+                            t.edge_coef = s2;
                         } else {
-                            if (ch < 3) { t.edge_src = ch + 1; t.edge_coef = ch + 1; }
-                            else t.fixed = fQA;
+                            t.fixed = term == 0 ? fK1A : fK2A;
+                            t.has_E = term == 1;
                         }
-                        tasks.push_back(t);
+                    } else if (!adj) {
+                        if (ch == 0) t.fixed = fQF;
+                        else { t.edge_src = ch - 1; t.edge_coef = ch + 1; }
+                        if (ch == 2) { t.trace_kind = 2; t.trace_coef = 8.0; }
+                    } else {
+                        if (ch < 3) { t.edge_src = ch + 1; t.edge_coef = ch + 1; }
+                        else t.fixed = fQA;
                     }
-            if (Sy.ntasks > 64) throw Fail{RCMPS_ERR_UNSUPPORTED};
-        }
-        P.D = D; P.Dp = Dp; P.npoints = npts;
-        P.a = std::sqrt(ip->max_point);
-        P.atol = ip->atol; P.rtol = ip->rtol;
-        P.h0 = ip->initial_step > 0 ? ip->initial_step : 0.01;
-        P.max_steps = ip->maxiters;
-        P.spin_limit = 20000000LL;
-        P.points = ctx->traj_points.d();
-        P.Q = st->Q.d(); P.Qh = st->Qh.d(); P.R = st->R.d(); P.Rh = st->Rh.d(); P.C = st->C.d();
-        P.tasks = nullptr;
-        P.abort_flag = static_cast<uint32_t*>(ctx->abort_word.p);
-        ck(ctx->traj_tasks.ensure(tasks.size() * sizeof(TrajTask)));
-        ck(ctx->traj_params.ensure(sizeof(TrajParams) + 4096 * 8));
-        // the Chebyshev table lives after the params
-        double* cheb = reinterpret_cast<double*>(static_cast<char*>(ctx->traj_params.p) + sizeof(TrajParams) + 64);
-        P.cheb = cheb;
-        P.tasks = static_cast<const TrajTask*>(ctx->traj_tasks.p);
-        ck(hipMemcpyAsync(cheb, RC_SOURCE_CHEB, sizeof(RC_SOURCE_CHEB), hipMemcpyHostToDevice, ctx->stream));
-        ck(hipMemcpyAsync(ctx->traj_tasks.p, tasks.data(), tasks.size() * sizeof(TrajTask), hipMemcpyHostToDevice,
-                          ctx->stream));
-        ck(hipMemcpyAsync(ctx->traj_params.p, &P, sizeof(TrajParams), hipMemcpyHostToDevice, ctx->stream));
-        ck(hipMemsetAsync(ctx->traj_flags.p, 0, flag_words * 4, ctx->stream));
-        ck(hipMemsetAsync(ctx->abort_word.p, 0, 256, ctx->stream));
-        ck(hipMemsetAsync(ctx->traj_out.p, 0, RC_NSYS * RC_OUT_STRIDE * 8, ctx->stream));
-        ck(hipEventRecord(ctx->ev[0], ctx->stream));
-        ck(rc_launch_trajectory(NB, (flags & RCMPS_FLAG_SPLIT) ? 1 : 0, (int)tasks.size(),
-                                static_cast<const TrajParams*>(ctx->traj_params.p),
-                                ctx->stream));
-        ck(hipEventRecord(ctx->ev[1], ctx->stream));
-        std::vector<double> outs(RC_NSYS * RC_OUT_STRIDE);
-        ck(hipMemcpyAsync(outs.data(), ctx->traj_out.d(), outs.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-        ck(hipStreamSynchronize(ctx->stream));
-        float tms = 0;
-        ck(hipEventElapsedTime(&tms, ctx->ev[0], ctx->ev[1]));
+                    tasks.push_back(t);
+                }
+        sys.push_back(Sy);
+    }
+}
+
+// Integrate every system of the given evaluations (all of one padded size Dp) in one persistent
+// launch; states are packed into rounds of at most the resident workgroup count.
+void run_class(rcmps_context* ctx, int Dp, bool split, const rcmps_integrator* ip, const std::vector<int>& idx,
+               std::vector<EvalReq>& reqs) {
+    const int NB = Dp / 16, npts = 2 * ip->n_quad + 1;
+    std::vector<TrajSys> sys;
+    std::vector<TrajTask> tasks;
+    while (ctx->tb.size() < idx.size()) ctx->tb.emplace_back(new TrajBufs());
+    std::vector<int> state_first, state_ntasks;
+    for (size_t j = 0; j < idx.size(); ++j) {
+        const int first = (int)tasks.size();
+        build_state(ctx, *ctx->tb[j], reqs[idx[j]].st, reqs[idx[j]].adjoint(), npts, sys, tasks);
+        state_first.push_back(first);
+        state_ntasks.push_back((int)tasks.size() - first);
+    }
+    int capacity = 0;
+    ck(rc_trajectory_capacity(NB, split ? 1 : 0, &capacity));
+    // gang schedule: a state's tasks share one round
+    std::vector<std::vector<int>> rounds;
+    int nblocks = 0;
+    for (size_t j = 0; j < idx.size(); ++j) {
+        if (state_ntasks[j] > capacity) throw Fail{RCMPS_ERR_UNSUPPORTED};
+        if (rounds.empty() || (int)rounds.back().size() + state_ntasks[j] > capacity) rounds.emplace_back();
+        for (int k = 0; k < state_ntasks[j]; ++k) rounds.back().push_back(state_first[j] + k);
+        nblocks = std::max(nblocks, (int)rounds.back().size());
+    }
+    std::vector<int> sched((size_t)rounds.size() * nblocks, -1);
+    for (size_t r = 0; r < rounds.size(); ++r)
+        for (size_t w = 0; w < rounds[r].size(); ++w) sched[r * nblocks + w] = rounds[r][w];
+
+    TrajParams P{};
+    P.Dp = Dp;
+    P.npoints = npts;
+    P.nrounds = (int)rounds.size();
+    P.a = std::sqrt(ip->max_point);
+    P.atol = ip->atol;
+    P.rtol = ip->rtol;
+    P.h0 = ip->initial_step > 0 ? ip->initial_step : 0.01;
+    P.max_steps = ip->maxiters;
+    P.spin_limit = 20000000LL;
+    P.points = ctx->traj_points.d();
+    P.abort_flag = static_cast<uint32_t*>(ctx->abort_word.p);
+    ck(ctx->traj_tasks.ensure(tasks.size() * sizeof(TrajTask)));
+    ck(ctx->traj_sys.ensure(sys.size() * sizeof(TrajSys)));
+    ck(ctx->traj_sched.ensure(sched.size() * sizeof(int)));
+    ck(ctx->traj_params.ensure(sizeof(TrajParams) + 64 + sizeof(RC_SOURCE_CHEB)));
+    double* cheb = reinterpret_cast<double*>(static_cast<char*>(ctx->traj_params.p) + sizeof(TrajParams) + 64);
+    P.cheb = cheb;
+    P.tasks = static_cast<const TrajTask*>(ctx->traj_tasks.p);
+    P.sys = static_cast<const TrajSys*>(ctx->traj_sys.p);
+    P.schedule = static_cast<const int*>(ctx->traj_sched.p);
+    ck(hipMemcpyAsync(cheb, RC_SOURCE_CHEB, sizeof(RC_SOURCE_CHEB), hipMemcpyHostToDevice, ctx->stream));
+    ck(hipMemcpyAsync(ctx->traj_tasks.p, tasks.data(), tasks.size() * sizeof(TrajTask), hipMemcpyHostToDevice,
+                      ctx->stream));
+    ck(hipMemcpyAsync(ctx->traj_sys.p, sys.data(), sys.size() * sizeof(TrajSys), hipMemcpyHostToDevice, ctx->stream));
+    ck(hipMemcpyAsync(ctx->traj_sched.p, sched.data(), sched.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    ck(hipMemcpyAsync(ctx->traj_params.p, &P, sizeof(TrajParams), hipMemcpyHostToDevice, ctx->stream));
+    for (size_t j = 0; j < idx.size(); ++j) {
+        ck(hipMemsetAsync(ctx->tb[j]->flags.p, 0, ctx->tb[j]->flag_bytes, ctx->stream));
+        ck(hipMemsetAsync(ctx->tb[j]->out.p, 0, RC_NSYS * RC_OUT_STRIDE * 8, ctx->stream));
+    }
+    ck(hipMemsetAsync(ctx->abort_word.p, 0, 256, ctx->stream));
+    ck(hipEventRecord(ctx->ev[0], ctx->stream));
+    ck(rc_launch_trajectory(NB, split ? 1 : 0, nblocks, static_cast<const TrajParams*>(ctx->traj_params.p),
+                            ctx->stream));
+    ck(hipEventRecord(ctx->ev[1], ctx->stream));
+    std::vector<double> outs(idx.size() * RC_NSYS * RC_OUT_STRIDE);
+    for (size_t j = 0; j < idx.size(); ++j)
+        ck(hipMemcpyAsync(outs.data() + j * RC_NSYS * RC_OUT_STRIDE, ctx->tb[j]->out.d(), RC_NSYS * RC_OUT_STRIDE * 8,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    ck(hipStreamSynchronize(ctx->stream));
+    float tms = 0;
+    ck(hipEventElapsedTime(&tms, ctx->ev[0], ctx->ev[1]));
+    for (size_t j = 0; j < idx.size(); ++j) {
+        EvalReq& r = reqs[idx[j]];
+        rcmps_stats* S = r.S;
         S->trajectory_ms = tms;
         S->status = RCMPS_OK;
         S->failed_system = -1;
         for (int s = 0; s < RC_NSYS; ++s) {
-            const double* o = outs.data() + s * RC_OUT_STRIDE;
+            const double* o = outs.data() + (j * RC_NSYS + s) * RC_OUT_STRIDE;
             S->attempts[s] = (long long)o[RC_OUT_ATTEMPTS];
             S->accepted[s] = (long long)o[RC_OUT_ACCEPTED];
             S->rejected[s] = (long long)o[RC_OUT_REJECTED];
             S->dense_steps[s] = (long long)o[RC_OUT_DENSE];
             S->rhs_calls[s] = (long long)o[RC_OUT_RHS];
             for (int q = 0; q < RC_NTIMERS; ++q) S->phase_us[s][q] = o[RC_OUT_TIMERS + q] / ctx->clock_mhz;
-            int stt = (int)o[RC_OUT_STATUS];
+            const int stt = (int)o[RC_OUT_STATUS];
             if (stt != RC_ST_OK && S->status == RCMPS_OK) { S->status = stt; S->failed_system = s; }
         }
-        ctx->last_D = D; ctx->last_Dp = Dp; ctx->last_npts = npts;
-        if (S->status != RCMPS_OK) throw Fail{S->status};
-        for (int term = 0; term < 3; ++term) S->term_values[term] = outs[(2 * term) * RC_OUT_STRIDE + RC_OUT_SCALAR];
-        const double coef[3] = {1.0, 1.0, g};
-        double E = 0;
-        for (int term = 0; term < 3; ++term) E += coef[term] * S->term_values[term];
-        *energy = E;
-        ctx->last_adjoint = !energy_only;
-        if (energy_only) return;
-
-        // ----------------------------------------------------------------------------------
-        // pullback (src/observables.jl:161-199) in reduced channels
-        // ----------------------------------------------------------------------------------
-        ck(hipEventRecord(ctx->ev[2], ctx->stream));
-        ck(ctx->tmpM.ensure((size_t)nn * mb));
-        ck(ctx->scratch.ensure(12 * mb));
-        double* M = ctx->tmpM.d();
-        double* cov = ctx->scratch.d();
-        double* env = cov + mcount;
-        double* T1 = cov + 2 * mcount;
-        double* T2 = cov + 3 * mcount;
-        double* U = cov + 4 * mcount;
-        double* X1 = cov + 5 * mcount;
-        double* X2 = cov + 6 * mcount;
-        double* cterm = cov + 7 * mcount;
-        double* tmp = cov + 8 * mcount;
-        double* tmp2 = cov + 9 * mcount;
-        // dE = 2 Re tr(XR dR) + 2 Re tr(XQ dQ) for any (dR, dQ); covector = XR - XQ R^+
-        double* XR = cov + 10 * mcount;
-        double* XQ = cov + 11 * mcount;
-        ck(hipMemsetAsync(env, 0, mb, ctx->stream));
-        ck(hipMemsetAsync(XR, 0, 2 * mb, ctx->stream));
-        const long long node = (long long)Dp * Dp;   // complex elements per matrix
-        for (int term = 0; term < 3; ++term) {
-            const int sf = 2 * term, sa = sf + 1;
-            const int lf = nlive[sf], la = nlive[sa];
-            const double* Fs = ctx->traj_samples[sf].d();
-            const double* As = ctx->traj_samples[sa].d();
-            const long long fstride = lf * node, astride = la * node;
-            // adjoint sample of node i is at index nn-1-i: base at nn-1, negative stride
-            auto Aptr = [&](int ch) { return As + 2 * ((long long)(nn - 1) * astride + ch * node); };
-            auto Fptr = [&](int ch) { return Fs + 2 * (ch * node); };
-            ck(hipMemsetAsync(T1, 0, 3 * mb, ctx->stream));   // T1, T2, U
-            if (term < 2) {
-                // k = 1 (F1 = rho):  T1 += rho R^+ (sum w A1), T2 += rho (sum w A1)
-                hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Aptr(0), -astride,
-                                   ctx->nodes_w.d(), nn, dv.n(), 0.0, X1);
-                hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Aptr(1), -astride,
-                                   ctx->nodes_j.d(), nn, dv.n(), 0.0, X2);           // sum j A2
-                hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Fptr(0), fstride,
-                                   ctx->nodes_j.d(), nn, dv.n(), 0.0, tmp2);         // sum j F2
-                ck(hipGetLastError());
-                dv.gemm(0, st->Rh.d(), 0, X1, tmp);
-                dv.gemm(0, st->rho.d(), 0, tmp, T1);
-                dv.gemm(0, st->rho.d(), 0, X1, T2);
-                // k = 2: F2 (w R^+ A2), w F2 A2 ; k = 3: F2^+ (w R^+ A2^+), w F2^+ A2^+
-                for (int k3 = 0; k3 < 2; ++k3) {
-                    GemmArgs gb{};
-                    gb.opA = 0; gb.A = st->Rh.d(); gb.strideA = 0;
-                    gb.opB = k3; gb.B = Aptr(1); gb.strideB = -astride;
-                    gb.C = M; gb.strideC = node;
-                    gb.alpha = ctx->nodes_w.d(); gb.alpha_stride = 1;
-                    dv.batched(gb, nn);
-                    GemmArgs gr{};
-                    gr.opA = k3; gr.A = Fptr(0); gr.strideA = fstride;
-                    gr.opB = 0; gr.B = M; gr.strideB = node;
-                    dv.reduce(gr, nn, T1, 1.0, 1.0);
-                    GemmArgs g2{};
-                    g2.opA = k3; g2.A = Fptr(0); g2.strideA = fstride;
-                    g2.opB = k3; g2.B = Aptr(1); g2.strideB = -astride;
-                    g2.alpha = ctx->nodes_w.d(); g2.alpha_stride = 1;
-                    dv.reduce(g2, nn, T2, 1.0, 1.0);
-                }
-                // u = sqrt2 (rho sum j A2 + (sum j F2)^+), the coefficient of the edge operator
-                dv.gemm(0, st->rho.d(), 0, X2, U, s2);
-                dv.herm(1, s2, tmp2, tmp);
-                dv.axpby(cd(1, 0), U, cd(1, 0), tmp, U);
-                if (term == 0) {
-                    dv.axpby(cd(1, 0), T1, cd(1, 0), U, T1);           // op = R
-                } else {
-                    // op = C = QR - RQ: dC = dQ R + Q dR - dR Q - R dQ
-                    dv.gemm(0, U, 0, st->Q.d(), T1, 1.0, 1.0);
-                    dv.gemm(0, st->Q.d(), 0, U, T1, -1.0, 1.0);
-                    dv.gemm(0, st->R.d(), 0, U, T2, 1.0, 1.0);
-                    dv.gemm(0, U, 0, st->R.d(), T2, -1.0, 1.0);
-                }
-            } else {
-                // phi^4: sum_k F_k (w R^+ A_k + j k A_{k+1}) - (sum_k w F_k A_k) R^+,  F_1 = rho, A_5 = I
-                for (int k = 1; k <= 4; ++k) {
-                    GemmArgs gb{};
-                    gb.opA = 0; gb.A = st->Rh.d(); gb.strideA = 0;
-                    gb.opB = 0; gb.B = Aptr(k - 1); gb.strideB = -astride;
-                    gb.C = M; gb.strideC = node;
-                    gb.alpha = ctx->nodes_w.d(); gb.alpha_stride = 1;
-                    gb.dalpha = ctx->nodes_j.d(); gb.dscale = k;
-                    if (k < 4) { gb.addD = Aptr(k); gb.strideD = -astride; }
-                    else gb.add_identity = 4.0;
-                    dv.batched(gb, nn);
-                    GemmArgs gr{};
-                    gr.opA = 0;
-                    gr.A = k == 1 ? st->rho.d() : Fptr(k - 2);
-                    gr.strideA = k == 1 ? 0 : fstride;
-                    gr.opB = 0; gr.B = M; gr.strideB = node;
-                    dv.reduce(gr, nn, T1, 1.0, 1.0);
-                    GemmArgs g2{};
-                    g2.opA = 0;
-                    g2.A = k == 1 ? st->rho.d() : Fptr(k - 2);
-                    g2.strideA = k == 1 ? 0 : fstride;
-                    g2.opB = 0; g2.B = Aptr(k - 1); g2.strideB = -astride;
-                    g2.alpha = ctx->nodes_w.d(); g2.alpha_stride = 1;
-                    dv.reduce(g2, nn, T2, 1.0, 1.0);
-                }
-            }
-            // environment: hermitised adjoint channel 1 at the endpoint
-            const double* aend = As + 2 * ((long long)nn * astride);
-            dv.herm(0, 1.0, aend, tmp);
-            const double c = coef[term];
-            dv.axpby(cd(1, 0), XR, cd(c, 0), T1, XR);
-            dv.axpby(cd(1, 0), XQ, cd(c, 0), T2, XQ);
-            dv.axpby(cd(1, 0), env, cd(c, 0), tmp, env);
+        if (S->status == RCMPS_OK) {
+            for (int term = 0; term < 3; ++term)
+                S->term_values[term] = outs[(j * RC_NSYS + 2 * term) * RC_OUT_STRIDE + RC_OUT_SCALAR];
+            const double coef[3] = {1.0, 1.0, r.g};
+            double E = 0;
+            for (int term = 0; term < 3; ++term) E += coef[term] * S->term_values[term];
+            *r.energy = E;
         }
-        auto covector = [&] {   // horizontal directions dQ = -R^+ W
-            dv.gemm(0, XQ, 0, st->Rh.d(), cterm);
-            dv.axpby(cd(1, 0), XR, cd(-1, 0), cterm, cov);
-        };
-        covector();
-        ck(hipEventRecord(ctx->ev[3], ctx->stream));
-        if (raw_cov) pad_out(ctx, D, Dp, cov, raw_cov);
-        if (raw_env) pad_out(ctx, D, Dp, env, raw_env);
-
-        // ----------------------------------------------------------------------------------
-        // finish_gradient (src/observables.jl:201-212)
-        // ----------------------------------------------------------------------------------
-        if (grad || grad_R || grad_K) {
-            if (!(flags & RCMPS_FLAG_NO_BOUNDARY)) {
-                ck(ctx->tmpM.ensure(2 * mb));
-                double* hb = X1;
-                // aims two digits below the reference's 1e-11 and accepts at its bar
-                boundary_solve(ctx, st, env, 1e-13, 1e-11, hb, S);
-                // rho0 responds to dL(rho0) = dQ rho0 + rho0 dQ^+ + dR rho0 R^+ + R rho0 dR^+:
-                // XR += rho R^+ H, XQ += rho H  (covector += rho R^+ H - rho H R^+)
-                dv.gemm(0, st->rho.d(), 0, st->Rh.d(), tmp);
-                dv.gemm(0, tmp, 0, hb, XR, 1.0, 1.0);
-                dv.gemm(0, st->rho.d(), 0, hb, XQ, 1.0, 1.0);
-                covector();
-            }
-        }
-        if (grad_R || grad_K) {
-            // Q = -iK - R^+R/2: dQ = -i dK - (dR^+ R + R^+ dR)/2, so
-            // dE = 2 Re tr[(XR - herm(XQ) R^+) dR] + 2 Re tr[herm(-i XQ) dK]; returned as the
-            // gradients G with dE = Re <G, d> = Re sum conj(G_ij) d_ij (K restricted to Hermitian)
-            if (grad_R) {
-                dv.herm(0, 1.0, XQ, tmp2);
-                dv.gemm(0, tmp2, 0, st->Rh.d(), tmp);
-                dv.axpby(cd(1, 0), XR, cd(-1, 0), tmp, tmp);
-                dv.herm(1, 2.0, tmp, tmp2);
-                pad_out(ctx, D, Dp, tmp2, grad_R);
-            }
-            if (grad_K) {
-                dv.herm(1, 1.0, XQ, tmp);
-                dv.axpby(cd(0, -1), XQ, cd(0, 1), tmp, tmp2);
-                pad_out(ctx, D, Dp, tmp2, grad_K);
-            }
-        }
-        if (grad) {
-            // G = [(rho + reg I)^{-1} cov]^+ = cov^+ (rho + reg I)^{-1}
-            rcmps_state* ms = const_cast<rcmps_state*>(st);
-            if (ms->metric_reg_cached != metric_reg) {
-                Mat a = st->rho_host;
-                for (int i = 0; i < D; ++i) a[(size_t)i * D + i] += metric_reg;
-                Mat inv;
-                if (!rc::inverse(D, a, inv)) throw Fail{RCMPS_ERR_SOLVER};
-                ck(ms->Minv.ensure(mb));
-                upload(ctx, D, Dp, inv, ms->Minv.d());
-                ms->metric_reg_cached = metric_reg;
-            }
-            dv.gemm(1, cov, 0, st->Minv.d(), tmp);
-            pad_out(ctx, D, Dp, tmp, grad);
-        }
-        ck(hipEventRecord(ctx->ev[4], ctx->stream));
-        ck(hipStreamSynchronize(ctx->stream));
-        float pms = 0, bms = 0;
-        ck(hipEventElapsedTime(&pms, ctx->ev[2], ctx->ev[3]));
-        ck(hipEventElapsedTime(&bms, ctx->ev[3], ctx->ev[4]));
-        S->pullback_ms = pms;
-        S->boundary_ms = bms;
-    });
-    S->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    if (rc && S->status == RCMPS_OK) S->status = rc;
-    return rc;
+    }
+    if (!idx.empty()) {
+        const rcmps_state* st0 = reqs[idx[0]].st;
+        ctx->last_D = st0->D; ctx->last_Dp = st0->Dp; ctx->last_npts = npts;
+        ctx->last_adjoint = reqs[idx[0]].adjoint();
+    }
 }
+
+// Pullback over the Gauss nodes, boundary correction and gradients of one evaluation whose
+// trajectories are in tb (src/observables.jl:161-212).
+void finish_state(rcmps_context* ctx, EvalReq& r, TrajBufs& tb, const rcmps_integrator* ip, double metric_reg,
+                  int flags) {
+    const rcmps_state* st = r.st;
+    rcmps_stats* S = r.S;
+    double *grad = r.grad, *raw_cov = r.raw_cov, *raw_env = r.raw_env, *grad_R = r.grad_R, *grad_K = r.grad_K;
+    const int D = st->D, Dp = st->Dp;
+    Dev dv{ctx, Dp, D};
+    const size_t mb = (size_t)Dp * Dp * 16;
+    const size_t mcount = (size_t)Dp * Dp * 2;
+    const int nn = 2 * ip->n_quad;
+    const double s2 = std::sqrt(2.0);
+    const double coef[3] = {1.0, 1.0, r.g};
+    // ----------------------------------------------------------------------------------
+    // pullback (src/observables.jl:161-199) in reduced channels
+    // ----------------------------------------------------------------------------------
+    ck(hipEventRecord(ctx->ev[2], ctx->stream));
+    ck(ctx->tmpM.ensure((size_t)nn * mb));
+    ck(ctx->scratch.ensure(12 * mb));
+    double* M = ctx->tmpM.d();
+    double* cov = ctx->scratch.d();
+    double* env = cov + mcount;
+    double* T1 = cov + 2 * mcount;
+    double* T2 = cov + 3 * mcount;
+    double* U = cov + 4 * mcount;
+    double* X1 = cov + 5 * mcount;
+    double* X2 = cov + 6 * mcount;
+    double* cterm = cov + 7 * mcount;
+    double* tmp = cov + 8 * mcount;
+    double* tmp2 = cov + 9 * mcount;
+    // dE = 2 Re tr(XR dR) + 2 Re tr(XQ dQ) for any (dR, dQ); covector = XR - XQ R^+
+    double* XR = cov + 10 * mcount;
+    double* XQ = cov + 11 * mcount;
+    ck(hipMemsetAsync(env, 0, mb, ctx->stream));
+    ck(hipMemsetAsync(XR, 0, 2 * mb, ctx->stream));
+    const long long node = (long long)Dp * Dp;   // complex elements per matrix
+    for (int term = 0; term < 3; ++term) {
+        const int sf = 2 * term, sa = sf + 1;
+        const int lf = kLive[sf], la = kLive[sa];
+        const double* Fs = tb.samples[sf].d();
+        const double* As = tb.samples[sa].d();
+        const long long fstride = lf * node, astride = la * node;
+        // adjoint sample of node i is at index nn-1-i: base at nn-1, negative stride
+        auto Aptr = [&](int ch) { return As + 2 * ((long long)(nn - 1) * astride + ch * node); };
+        auto Fptr = [&](int ch) { return Fs + 2 * (ch * node); };
+        ck(hipMemsetAsync(T1, 0, 3 * mb, ctx->stream));   // T1, T2, U
+        if (term < 2) {
+            // k = 1 (F1 = rho):  T1 += rho R^+ (sum w A1), T2 += rho (sum w A1)
+            hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Aptr(0), -astride,
+                               ctx->nodes_w.d(), nn, dv.n(), 0.0, X1);
+            hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Aptr(1), -astride,
+                               ctx->nodes_j.d(), nn, dv.n(), 0.0, X2);           // sum j A2
+            hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Fptr(0), fstride,
+                               ctx->nodes_j.d(), nn, dv.n(), 0.0, tmp2);         // sum j F2
+            ck(hipGetLastError());
+            dv.gemm(0, st->Rh.d(), 0, X1, tmp);
+            dv.gemm(0, st->rho.d(), 0, tmp, T1);
+            dv.gemm(0, st->rho.d(), 0, X1, T2);
+            // k = 2: F2 (w R^+ A2), w F2 A2 ; k = 3: F2^+ (w R^+ A2^+), w F2^+ A2^+
+            for (int k3 = 0; k3 < 2; ++k3) {
+                GemmArgs gb{};
+                gb.opA = 0; gb.A = st->Rh.d(); gb.strideA = 0;
+                gb.opB = k3; gb.B = Aptr(1); gb.strideB = -astride;
+                gb.C = M; gb.strideC = node;
+                gb.alpha = ctx->nodes_w.d(); gb.alpha_stride = 1;
+                dv.batched(gb, nn);
+                GemmArgs gr{};
+                gr.opA = k3; gr.A = Fptr(0); gr.strideA = fstride;
+                gr.opB = 0; gr.B = M; gr.strideB = node;
+                dv.reduce(gr, nn, T1, 1.0, 1.0);
+                GemmArgs g2{};
+                g2.opA = k3; g2.A = Fptr(0); g2.strideA = fstride;
+                g2.opB = k3; g2.B = Aptr(1); g2.strideB = -astride;
+                g2.alpha = ctx->nodes_w.d(); g2.alpha_stride = 1;
+                dv.reduce(g2, nn, T2, 1.0, 1.0);
+            }
+            // u = sqrt2 (rho sum j A2 + (sum j F2)^+), the coefficient of the edge operator
+            dv.gemm(0, st->rho.d(), 0, X2, U, s2);
+            dv.herm(1, s2, tmp2, tmp);
+            dv.axpby(cd(1, 0), U, cd(1, 0), tmp, U);
+            if (term == 0) {
+                dv.axpby(cd(1, 0), T1, cd(1, 0), U, T1);           // op = R
+            } else {
+                // op = C = QR - RQ: dC = dQ R + Q dR - dR Q - R dQ
+                dv.gemm(0, U, 0, st->Q.d(), T1, 1.0, 1.0);
+                dv.gemm(0, st->Q.d(), 0, U, T1, -1.0, 1.0);
+                dv.gemm(0, st->R.d(), 0, U, T2, 1.0, 1.0);
+                dv.gemm(0, U, 0, st->R.d(), T2, -1.0, 1.0);
+            }
+        } else {
+            // phi^4: sum_k F_k (w R^+ A_k + j k A_{k+1}) - (sum_k w F_k A_k) R^+,  F_1 = rho, A_5 = I
+            for (int k = 1; k <= 4; ++k) {
+                GemmArgs gb{};
+                gb.opA = 0; gb.A = st->Rh.d(); gb.strideA = 0;
+                gb.opB = 0; gb.B = Aptr(k - 1); gb.strideB = -astride;
+                gb.C = M; gb.strideC = node;
+                gb.alpha = ctx->nodes_w.d(); gb.alpha_stride = 1;
+                gb.dalpha = ctx->nodes_j.d(); gb.dscale = k;
+                if (k < 4) { gb.addD = Aptr(k); gb.strideD = -astride; }
+                else gb.add_identity = 4.0;
+                dv.batched(gb, nn);
+                GemmArgs gr{};
+                gr.opA = 0;
+                gr.A = k == 1 ? st->rho.d() : Fptr(k - 2);
+                gr.strideA = k == 1 ? 0 : fstride;
+                gr.opB = 0; gr.B = M; gr.strideB = node;
+                dv.reduce(gr, nn, T1, 1.0, 1.0);
+                GemmArgs g2{};
+                g2.opA = 0;
+                g2.A = k == 1 ? st->rho.d() : Fptr(k - 2);
+                g2.strideA = k == 1 ? 0 : fstride;
+                g2.opB = 0; g2.B = Aptr(k - 1); g2.strideB = -astride;
+                g2.alpha = ctx->nodes_w.d(); g2.alpha_stride = 1;
+                dv.reduce(g2, nn, T2, 1.0, 1.0);
+            }
+        }
+        // environment: hermitised adjoint channel 1 at the endpoint
+        const double* aend = As + 2 * ((long long)nn * astride);
+        dv.herm(0, 1.0, aend, tmp);
+        const double c = coef[term];
+        dv.axpby(cd(1, 0), XR, cd(c, 0), T1, XR);
+        dv.axpby(cd(1, 0), XQ, cd(c, 0), T2, XQ);
+        dv.axpby(cd(1, 0), env, cd(c, 0), tmp, env);
+    }
+    auto covector = [&] {   // horizontal directions dQ = -R^+ W
+        dv.gemm(0, XQ, 0, st->Rh.d(), cterm);
+        dv.axpby(cd(1, 0), XR, cd(-1, 0), cterm, cov);
+    };
+    covector();
+    ck(hipEventRecord(ctx->ev[3], ctx->stream));
+    if (raw_cov) pad_out(ctx, D, Dp, cov, raw_cov);
+    if (raw_env) pad_out(ctx, D, Dp, env, raw_env);
+
+    // ----------------------------------------------------------------------------------
+    // finish_gradient (src/observables.jl:201-212)
+    // ----------------------------------------------------------------------------------
+    if (grad || grad_R || grad_K) {
+        if (!(flags & RCMPS_FLAG_NO_BOUNDARY)) {
+            ck(ctx->tmpM.ensure(2 * mb));
+            double* hb = X1;
+            // aims two digits below the reference's 1e-11 and accepts at its bar
+            boundary_solve(ctx, st, env, 1e-13, 1e-11, hb, S);
+            // rho0 responds to dL(rho0) = dQ rho0 + rho0 dQ^+ + dR rho0 R^+ + R rho0 dR^+:
+            // XR += rho R^+ H, XQ += rho H  (covector += rho R^+ H - rho H R^+)
+            dv.gemm(0, st->rho.d(), 0, st->Rh.d(), tmp);
+            dv.gemm(0, tmp, 0, hb, XR, 1.0, 1.0);
+            dv.gemm(0, st->rho.d(), 0, hb, XQ, 1.0, 1.0);
+            covector();
+        }
+    }
+    if (grad_R || grad_K) {
+        // Q = -iK - R^+R/2: dQ = -i dK - (dR^+ R + R^+ dR)/2, so
+        // dE = 2 Re tr[(XR - herm(XQ) R^+) dR] + 2 Re tr[herm(-i XQ) dK]; returned as the
+        // gradients G with dE = Re <G, d> = Re sum conj(G_ij) d_ij (K restricted to Hermitian)
+        if (grad_R) {
+            dv.herm(0, 1.0, XQ, tmp2);
+            dv.gemm(0, tmp2, 0, st->Rh.d(), tmp);
+            dv.axpby(cd(1, 0), XR, cd(-1, 0), tmp, tmp);
+            dv.herm(1, 2.0, tmp, tmp2);
+            pad_out(ctx, D, Dp, tmp2, grad_R);
+        }
+        if (grad_K) {
+            dv.herm(1, 1.0, XQ, tmp);
+            dv.axpby(cd(0, -1), XQ, cd(0, 1), tmp, tmp2);
+            pad_out(ctx, D, Dp, tmp2, grad_K);
+        }
+    }
+    if (grad) {
+        // G = [(rho + reg I)^{-1} cov]^+ = cov^+ (rho + reg I)^{-1}
+        rcmps_state* ms = const_cast<rcmps_state*>(st);
+        if (ms->metric_reg_cached != metric_reg) {
+            Mat a = st->rho_host;
+            for (int i = 0; i < D; ++i) a[(size_t)i * D + i] += metric_reg;
+            Mat inv;
+            if (!rc::inverse(D, a, inv)) throw Fail{RCMPS_ERR_SOLVER};
+            ck(ms->Minv.ensure(mb));
+            upload(ctx, D, Dp, inv, ms->Minv.d());
+            ms->metric_reg_cached = metric_reg;
+        }
+        dv.gemm(1, cov, 0, st->Minv.d(), tmp);
+        pad_out(ctx, D, Dp, tmp, grad);
+    }
+    ck(hipEventRecord(ctx->ev[4], ctx->stream));
+    ck(hipStreamSynchronize(ctx->stream));
+    float pms = 0, bms = 0;
+    ck(hipEventElapsedTime(&pms, ctx->ev[2], ctx->ev[3]));
+    ck(hipEventElapsedTime(&bms, ctx->ev[3], ctx->ev[4]));
+    S->pullback_ms = pms;
+    S->boundary_ms = bms;
+}
+
+bool valid_integrator(const rcmps_integrator* ip) {
+    return ip && ip->n_quad >= 8 && ip->max_point > 0 && ip->atol > 0 && ip->rtol > 0 && ip->maxiters >= 1;
+}
+
+// A batch of evaluations: grouped by padded size, one persistent launch per group, then the
+// pullback of each evaluation.  Every request gets its stats; the first failure is returned.
+int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcmps_integrator* ip,
+                       double metric_reg, int flags) {
+    if (!ctx || !valid_integrator(ip) || !(metric_reg >= 0)) return RCMPS_ERR_ARGUMENT;
+    if (std::sqrt(ip->max_point) >= RC_SOURCE_TMAX) return RCMPS_ERR_UNSUPPORTED;
+    for (auto& r : reqs) {
+        if (!r.st || !r.energy || !r.S || !std::isfinite(r.g)) return RCMPS_ERR_ARGUMENT;
+        if (r.st->Dp / 16 > 4) return RCMPS_ERR_UNSUPPORTED;
+        std::memset(r.S, 0, sizeof(*r.S));
+    }
+    auto t_start = std::chrono::steady_clock::now();
+    int rc = guard([&] {
+        ck(hipSetDevice(ctx->device));
+        ensure_quadrature(ctx, ip);
+        for (int Dp = 16; Dp <= 64; Dp += 16) {
+            std::vector<int> idx;
+            for (size_t i = 0; i < reqs.size(); ++i)
+                if (reqs[i].st->Dp == Dp) idx.push_back((int)i);
+            if (idx.empty()) continue;
+            run_class(ctx, Dp, (flags & RCMPS_FLAG_SPLIT) != 0, ip, idx, reqs);
+            for (size_t j = 0; j < idx.size(); ++j) {
+                EvalReq& r = reqs[idx[j]];
+                if (r.S->status != RCMPS_OK || !r.adjoint()) continue;
+                const int code = guard([&] { finish_state(ctx, r, *ctx->tb[j], ip, metric_reg, flags); });
+                if (code != RCMPS_OK) r.S->status = code;
+            }
+        }
+    });
+    const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    for (auto& r : reqs) {
+        r.S->total_ms = total;
+        if (rc && r.S->status == RCMPS_OK) r.S->status = rc;
+    }
+    if (rc) return rc;
+    for (auto& r : reqs)
+        if (r.S->status != RCMPS_OK) return r.S->status;
+    return RCMPS_OK;
+}
+
+int fg_phi4_impl(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip,
+                 double metric_reg, int flags, double* energy, double* grad, double* raw_cov, double* raw_env,
+                 double* grad_R, double* grad_K, rcmps_stats* stats) {
+    if (!ctx || !st || !ip || !energy) return RCMPS_ERR_ARGUMENT;
+    rcmps_stats local{};
+    std::vector<EvalReq> reqs(1);
+    reqs[0].st = st; reqs[0].g = g; reqs[0].energy = energy;
+    reqs[0].grad = grad; reqs[0].raw_cov = raw_cov; reqs[0].raw_env = raw_env;
+    reqs[0].grad_R = grad_R; reqs[0].grad_K = grad_K;
+    reqs[0].S = stats ? stats : &local;
+    return fg_phi4_batch_impl(ctx, reqs, ip, metric_reg, flags);
+}
+
+}  // namespace
 
 int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip,
                   double metric_reg, int flags, double* energy, double* grad, double* raw_cov, double* raw_env,
@@ -1113,19 +1245,36 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
                         stats);
 }
 
+int rcmps_fg_phi4_batch(rcmps_context* ctx, int n, const rcmps_state* const* states, const double* g,
+                        const rcmps_integrator* ip, double metric_reg, int flags, double* energies,
+                        double* const* grads, rcmps_stats* stats) {
+    if (!ctx || n < 1 || !states || !g || !energies) return RCMPS_ERR_ARGUMENT;
+    std::vector<rcmps_stats> local(stats ? 0 : n);
+    std::vector<EvalReq> reqs(n);
+    for (int i = 0; i < n; ++i) {
+        reqs[i].st = states[i];
+        reqs[i].g = g[i];
+        reqs[i].energy = energies + i;
+        reqs[i].grad = grads ? grads[i] : nullptr;
+        reqs[i].S = stats ? stats + i : &local[i];
+    }
+    return fg_phi4_batch_impl(ctx, reqs, ip, metric_reg, flags);
+}
+
 int rcmps_fg_phi4_rk(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip, int flags,
                      double* energy, double* grad_R, double* grad_K, rcmps_stats* stats) {
     return fg_phi4_impl(ctx, st, g, ip, 0.0, flags, energy, nullptr, nullptr, nullptr, grad_R, grad_K, stats);
 }
 
 int rcmps_trajectory_samples(rcmps_context* ctx, int system, double* dst, long long capacity) {
-    if (!ctx || system < 0 || system >= RC_NSYS || !dst || ctx->last_Dp == 0) return RCMPS_ERR_ARGUMENT;
+    if (!ctx || system < 0 || system >= RC_NSYS || !dst || ctx->last_Dp == 0 || ctx->tb.empty())
+        return RCMPS_ERR_ARGUMENT;
     if (system % 2 && !ctx->last_adjoint) return RCMPS_ERR_ARGUMENT;
-    const int D = ctx->last_D, Dp = ctx->last_Dp, nl = ctx->nlive[system];
+    const int D = ctx->last_D, Dp = ctx->last_Dp, nl = kLive[system];
     const long long need = (long long)ctx->last_npts * nl * D * D * 2;
     if (capacity < need) return RCMPS_ERR_ARGUMENT;
     return guard([&] {
-        const double* src = ctx->traj_samples[system].d();
+        const double* src = ctx->tb[0]->samples[system].d();
         const int rows = ctx->last_npts * nl;
         for (int r = 0; r < rows; ++r)
             pad_out(ctx, D, Dp, src + (size_t)r * Dp * Dp * 2, dst + (size_t)r * D * D * 2);

@@ -32,45 +32,54 @@ __host__ __device__ __forceinline__ int rc_split_scale(double m) {
     return m > 0.0 ? 45 - ilogb(m) : 0;
 }
 
-// x 2^S (|x 2^S| < 2^46) -> digits s_0..s_2 in bytes 2,1,0 of hd and s_3..s_5 in bytes 2,1,0 of ld
-__host__ __device__ __forceinline__ void rc_digits(double x, int S, uint32_t& hd, uint32_t& ld) {
-    const double xs = ldexp(x, S - 24);          // |xs| < 2^22
-    int hi = (int)xs;                             // truncation; the remainder keeps the sign of xs
-    const int lo = (int)rint(ldexp(xs - (double)hi, 24));   // |lo| <= 2^24
-    const int lo2 = lo + 0x808080;
-    hi += lo2 >> 24;                              // carry of the balanced low digits, in {-1, 0, 1}
-    const int hi2 = hi + 0x808080;                // in (0, 2^24): balanced high digits, no carry out
-    hd = ((uint32_t)hi2 & 0xFFFFFFu) ^ 0x808080u;
-    ld = ((uint32_t)lo2 & 0xFFFFFFu) ^ 0x808080u;
+// 2^52 + B with B = 0x808080808080 (128 in each of six bytes)
+#define RC_SPLIT_MAGIC 4644889027444864.0
+
+// x 2^S -> the 48-bit integer X = rint(x 2^S) in six balanced digits.  fma(x, 2^S, 2^52 + B) rounds
+// once to the integer 2^52 + B + X (|X| < 2^46, so the sum stays in [2^52, 2^53) where the ulp is
+// 1); its low 48 mantissa bits are Y = X + B, and byte b of Y XOR 0x80 is the digit of weight
+// 256^b.  lo holds bytes 0-3 (digits s_5..s_2), bytes 0-1 of hi hold s_1, s_0.
+__host__ __device__ __forceinline__ void rc_digits(double x, double scale, uint32_t& lo, uint32_t& hi) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, fma(x, scale, RC_SPLIT_MAGIC));
+    lo = (uint32_t)u ^ 0x80808080u;
+    hi = (uint32_t)(u >> 32) ^ 0x8080u;
 }
 
-// byte q (0..3) of digit d of an element from its packed (hd, ld)
-__host__ __device__ __forceinline__ uint32_t rc_digit_byte(uint32_t hd, uint32_t ld, int d) {
-    const uint32_t w = d < 3 ? hd : ld;
-    const int sh = 8 * (2 - (d < 3 ? d : d - 3));
-    return (w >> sh) & 0xFFu;
+// byte d (0..3) of each of w0..w3 gathered into out[d] (v_perm_b32: selector bytes 0-3 address the
+// second operand, 4-7 the first)
+__device__ __forceinline__ void rc_bytes_t4(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t* out) {
+    const uint32_t a_lo = __builtin_amdgcn_perm(w1, w0, 0x05010400u), a_hi = __builtin_amdgcn_perm(w1, w0, 0x07030602u);
+    const uint32_t b_lo = __builtin_amdgcn_perm(w3, w2, 0x05010400u), b_hi = __builtin_amdgcn_perm(w3, w2, 0x07030602u);
+    out[0] = __builtin_amdgcn_perm(b_lo, a_lo, 0x05040100u);
+    out[1] = __builtin_amdgcn_perm(b_lo, a_lo, 0x07060302u);
+    out[2] = __builtin_amdgcn_perm(b_hi, a_hi, 0x05040100u);
+    out[3] = __builtin_amdgcn_perm(b_hi, a_hi, 0x07060302u);
+}
+__device__ __forceinline__ void rc_bytes_t2(uint32_t h0, uint32_t h1, uint32_t h2, uint32_t h3, uint32_t* out) {
+    const uint32_t a = __builtin_amdgcn_perm(h1, h0, 0x05010400u), b = __builtin_amdgcn_perm(h3, h2, 0x05010400u);
+    out[0] = __builtin_amdgcn_perm(b, a, 0x05040100u);
+    out[1] = __builtin_amdgcn_perm(b, a, 0x07060302u);
 }
 
 // Write four consecutive-k elements (re and im) of row/column `i` at k = k0..k0+3 into the
-// panel's digit planes (one 32-bit LDS store per digit and part).
-__device__ __forceinline__ void rc_store_digits4(unsigned char* panel, int i, int k0, const cplx* v, int S) {
-    uint32_t hr[4], lr[4], hm[4], lm[4];
+// panel's digit planes: one 32-bit LDS store per digit and part.  scale = 2^S.
+__device__ __forceinline__ void rc_store_digits4(unsigned char* panel, int i, int k0, const cplx* v, double scale) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        rc_digits(v[q].re, S, hr[q], lr[q]);
-        rc_digits(v[q].im, S, hm[q], lm[q]);
-    }
+    for (int part = 0; part < 2; ++part) {
+        uint32_t lo[4], hi[4];
 #pragma unroll
-    for (int d = 0; d < RC_SPLIT_DIGITS; ++d) {
-        uint32_t wr = 0, wi = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            wr |= rc_digit_byte(hr[q], lr[q], d) << (8 * q);
-            wi |= rc_digit_byte(hm[q], lm[q], d) << (8 * q);
-        }
-        unsigned char* base = panel + (size_t)(d * 2) * RC_PLANE_BYTES + i * RC_PLANE_STRIDE + k0;
-        *reinterpret_cast<uint32_t*>(base) = wr;
-        *reinterpret_cast<uint32_t*>(base + RC_PLANE_BYTES) = wi;
+        for (int q = 0; q < 4; ++q) rc_digits(part ? v[q].im : v[q].re, scale, lo[q], hi[q]);
+        uint32_t low[4], high[2];
+        rc_bytes_t4(lo[0], lo[1], lo[2], lo[3], low);     // low[b]: digit of weight 256^b
+        rc_bytes_t2(hi[0], hi[1], hi[2], hi[3], high);    // high[b]: weight 256^(4+b)
+        unsigned char* base = panel + (size_t)part * RC_PLANE_BYTES + i * RC_PLANE_STRIDE + k0;
+        // plane d holds the digit of weight 256^(5-d)
+        *reinterpret_cast<uint32_t*>(base + 0 * 2 * RC_PLANE_BYTES) = high[1];
+        *reinterpret_cast<uint32_t*>(base + 1 * 2 * RC_PLANE_BYTES) = high[0];
+        *reinterpret_cast<uint32_t*>(base + 2 * 2 * RC_PLANE_BYTES) = low[3];
+        *reinterpret_cast<uint32_t*>(base + 3 * 2 * RC_PLANE_BYTES) = low[2];
+        *reinterpret_cast<uint32_t*>(base + 4 * 2 * RC_PLANE_BYTES) = low[1];
+        *reinterpret_cast<uint32_t*>(base + 5 * 2 * RC_PLANE_BYTES) = low[0];
     }
 }
 

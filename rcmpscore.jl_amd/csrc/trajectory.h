@@ -31,8 +31,9 @@ struct TrajTask {
 };
 
 struct TrajSys {
-    int nlive, has_scalar, adj, first_task, ntasks, pad0;
+    int nlive, has_scalar, adj, first_task, ntasks, D;
     double n_elems;          // nlive * D^2 + has_scalar: denominator of the RMS norm
+    const double *Q, *Qh, *R, *Rh, *C;   // the state's matrices, Dp x Dp
     double* Y;               // [16][nlive][Dp*Dp] complex
     double* P;               // [16][nlive][Dp*Dp] complex
     double* E;               // [16][Dp*Dp] complex or null
@@ -45,16 +46,19 @@ struct TrajSys {
     double* out;             // [32]: status/counters, then phase timers of workgroup 0
 };
 
+// One launch integrates the systems of one or more states of the same padded size Dp.  Tasks
+// are gang-scheduled in rounds: workgroup w runs task schedule[r * nblocks + w] (or idles, -1)
+// in round r, and all tasks of a system share a round, so every wait is on a running partner.
 struct TrajParams {
-    int D, Dp, npoints, pad0;
+    int Dp, npoints, nrounds, pad0;
     double a, atol, rtol, h0;
     long long max_steps, spin_limit;
     const double* points;
-    const double *Q, *Qh, *R, *Rh, *C;
     const double* cheb;
     const TrajTask* tasks;
+    const TrajSys* sys;
+    const int* schedule;     // [nrounds][nblocks]
     uint32_t* abort_flag;
-    TrajSys sys[RC_NSYS];
 };
 
 // out[] layout per system

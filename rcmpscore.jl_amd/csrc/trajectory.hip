@@ -200,14 +200,16 @@ __device__ __forceinline__ void load_row_col_coh(cplx* M, cplx* N, const double*
 }
 
 
-// The task of this workgroup through scalar loads, so that every field is known wave-uniform.
-static_assert(sizeof(TrajTask) % 8 == 0, "task records are copied in 8-byte words");
-__device__ __forceinline__ TrajTask load_task(const TrajTask* tasks, int i) {
-    const __attribute__((address_space(4))) uint64_t* q = (const __attribute__((address_space(4))) uint64_t*)(tasks + i);
-    TrajTask t;
+// Record i of a read-only array through scalar loads, so that every field is known wave-uniform.
+template <typename T>
+__device__ __forceinline__ T load_scalar(const T* arr, int i) {
+    static_assert(sizeof(T) % 8 == 0, "records are copied in 8-byte words");
+    const __attribute__((address_space(4))) uint64_t* q =
+        (const __attribute__((address_space(4))) uint64_t*)(arr + __builtin_amdgcn_readfirstlane(i));
+    T t;
     uint64_t* w = reinterpret_cast<uint64_t*>(&t);
 #pragma unroll
-    for (int k = 0; k < (int)(sizeof(TrajTask) / 8); ++k) w[k] = q[k];
+    for (int k = 0; k < (int)(sizeof(T) / 8); ++k) w[k] = q[k];
     return t;
 }
 
@@ -288,12 +290,12 @@ __device__ __forceinline__ void fetch_col4(cplx* v, const double* g, uint32_t gb
 template <int NB>
 __device__ __forceinline__ void store_row4(unsigned char* panel, const cplx* v, int S) {
     const int t = threadIdx.x, k0 = 4 * (t & 15);
-    if (k0 < 16 * NB) rc_store_digits4(panel, t >> 4, k0, v, S);
+    if (k0 < 16 * NB) rc_store_digits4(panel, t >> 4, k0, v, ldexp(1.0, S));
 }
 template <int NB>
 __device__ __forceinline__ void store_col4(unsigned char* panel, const cplx* v, int S) {
     const int t = threadIdx.x, k0 = 4 * (t >> 4);
-    if (k0 < 16 * NB) rc_store_digits4(panel, t & 15, k0, v, S);
+    if (k0 < 16 * NB) rc_store_digits4(panel, t & 15, k0, v, ldexp(1.0, S));
 }
 
 // one constant panel (row block b or column block c of a fixed matrix) at kernel start
@@ -326,10 +328,9 @@ __device__ __forceinline__ cplx reduce_split(const double* part, double* red, in
 }  // namespace
 
 template <int NB, bool SPLIT>
-__global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __restrict__ pp) {
+__device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw) {
     using L = Lay<NB>;
     constexpr int Dp = L::Dp, NT = NB * NB;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx* sm = reinterpret_cast<cplx*>(smem_raw);
     cplx* Xr = sm + L::XR;
     cplx* Xc = sm + L::XC;
@@ -356,15 +357,14 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     __shared__ uint32_t* s_addr[64];
     __shared__ double s_tot[16];
 
-    const TrajParams& p = *pp;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int spa = (wave == 1 || wave == 3), spb = (wave == 1 || wave == 2);   // this wave's real product
     const int ei = tid >> 4, ej = tid & 15;
-    const TrajTask tk = load_task(p.tasks, blockIdx.x);
-    const TrajSys S = p.sys[tk.sys];
+    const TrajTask tk = load_scalar<TrajTask>(p.tasks, task);
+    const TrajSys S = load_scalar<TrajSys>(p.sys, tk.sys);
     const int b = tk.b, c = tk.c, ch = tk.ch, adj = tk.adj;
     const int row = 16 * b + ei, col = 16 * c + ej;
-    const bool real_el = row < p.D && col < p.D;
+    const bool real_el = row < S.D && col < S.D;
     const Ctl ctl{p.abort_flag, p.spin_limit};
     const uint32_t mat_bytes = (uint32_t)(Dp * Dp * 16);
     const uint32_t stage_stride = (uint32_t)S.nlive * mat_bytes;
@@ -382,28 +382,28 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
         for (int i = tid; i < 8 * RC_PANEL_BYTES / 16; i += 256) reinterpret_cast<v4i*>(smem_raw)[i] = v4i{0, 0, 0, 0};
         __syncthreads();
         if (!adj) {
-            sK0 = const_panel<NB>(pK0, p.R, true, b, scratch);
-            sK1 = const_panel<NB>(pK1, p.Q, true, b, scratch);
-            sK2 = const_panel<NB>(pK2, p.Qh, false, c, scratch);
-            sK3 = const_panel<NB>(pK3, p.Rh, false, c, scratch);
+            sK0 = const_panel<NB>(pK0, S.R, true, b, scratch);
+            sK1 = const_panel<NB>(pK1, S.Q, true, b, scratch);
+            sK2 = const_panel<NB>(pK2, S.Qh, false, c, scratch);
+            sK3 = const_panel<NB>(pK3, S.Rh, false, c, scratch);
         } else {
-            sK0 = const_panel<NB>(pK0, p.R, false, c, scratch);
-            sK1 = const_panel<NB>(pK1, p.Qh, true, b, scratch);
-            sK2 = const_panel<NB>(pK2, p.Q, false, c, scratch);
-            sK3 = const_panel<NB>(pK3, p.Rh, true, b, scratch);
-            if (tk.has_E) sK4 = const_panel<NB>(pK4, p.C, false, c, scratch);
+            sK0 = const_panel<NB>(pK0, S.R, false, c, scratch);
+            sK1 = const_panel<NB>(pK1, S.Qh, true, b, scratch);
+            sK2 = const_panel<NB>(pK2, S.Q, false, c, scratch);
+            sK3 = const_panel<NB>(pK3, S.Rh, true, b, scratch);
+            if (tk.has_E) sK4 = const_panel<NB>(pK4, S.C, false, c, scratch);
         }
     } else if (!adj) {
-        load_rowblk<NB, false>(Kl1, p.R, 0, b);
-        load_rowblk<NB, false>(Kq, p.Q, 0, b);
-        load_colblk<NB, false>(Kx, p.Qh, 0, c);
-        load_colblk<NB, false>(Kp, p.Rh, 0, c);
+        load_rowblk<NB, false>(Kl1, S.R, 0, b);
+        load_rowblk<NB, false>(Kq, S.Q, 0, b);
+        load_colblk<NB, false>(Kx, S.Qh, 0, c);
+        load_colblk<NB, false>(Kp, S.Rh, 0, c);
     } else {
-        load_colblk<NB, false>(Kl1, p.R, 0, c);
-        load_rowblk<NB, false>(Kq, p.Qh, 0, b);
-        load_colblk<NB, false>(Kx, p.Q, 0, c);
-        load_rowblk<NB, false>(Kp, p.Rh, 0, b);
-        if (tk.has_E) load_colblk<NB, false>(Kc, p.C, 0, c);
+        load_colblk<NB, false>(Kl1, S.R, 0, c);
+        load_rowblk<NB, false>(Kq, S.Qh, 0, b);
+        load_colblk<NB, false>(Kx, S.Q, 0, c);
+        load_rowblk<NB, false>(Kp, S.Rh, 0, b);
+        if (tk.has_E) load_colblk<NB, false>(Kc, S.C, 0, c);
     }
     cplx fixed_e = cmake(0, 0), trop_e = cmake(0, 0);
     if (tk.fixed) fixed_e = reinterpret_cast<const cplx*>(tk.fixed)[row * Dp + col];
@@ -644,7 +644,7 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     int next_point = 0, rej = 0, status = RC_ST_OK;
     bool alive = true;
     const __amdgpu_buffer_rsrc_t rS = rc_rsrc(S.samples, (uint32_t)((size_t)p.npoints * S.nlive * mat_bytes));
-    const int my_rank = blockIdx.x - S.first_task;
+    const int my_rank = task - S.first_task;
 
     int slot = 0;
     cplx Yin = y;
@@ -846,6 +846,19 @@ __global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __rest
     }
 }
 
+template <int NB, bool SPLIT>
+__global__ void __launch_bounds__(256, 1) rc_trajectory(const TrajParams* __restrict__ pp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const TrajParams& p = *pp;
+    const __attribute__((address_space(4))) int* sched = (const __attribute__((address_space(4))) int*)p.schedule;
+    for (int r = 0; r < p.nrounds; ++r) {
+        const int task = __builtin_amdgcn_readfirstlane(sched[r * (int)gridDim.x + (int)blockIdx.x]);
+        if (task < 0) continue;
+        run_task<NB, SPLIT>(p, task, smem_raw);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // host launcher
 // ---------------------------------------------------------------------------------------
@@ -860,21 +873,40 @@ int rc_trajectory_smem_bytes(int nb, int split) {
     }
 }
 
+static const void* trajectory_fn(int nb, int split) {
+    switch (nb * 2 + (split ? 1 : 0)) {
+        case 2: return (const void*)rc_trajectory<1, false>;
+        case 3: return (const void*)rc_trajectory<1, true>;
+        case 4: return (const void*)rc_trajectory<2, false>;
+        case 5: return (const void*)rc_trajectory<2, true>;
+        case 6: return (const void*)rc_trajectory<3, false>;
+        case 7: return (const void*)rc_trajectory<3, true>;
+        case 8: return (const void*)rc_trajectory<4, false>;
+        case 9: return (const void*)rc_trajectory<4, true>;
+        default: return nullptr;
+    }
+}
+
+// workgroups of the trajectory kernel that can be resident at once on the current device
+hipError_t rc_trajectory_capacity(int nb, int split, int* blocks) {
+    const void* fn = trajectory_fn(nb, split);
+    if (!fn) return hipErrorInvalidValue;
+    const int bytes = rc_trajectory_smem_bytes(nb, split);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    int per_cu = 0, dev = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, bytes)) != hipSuccess) return e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    *blocks = per_cu * cus;
+    return hipSuccess;
+}
+
 hipError_t rc_launch_trajectory(int nb, int split, int nblocks, const TrajParams* dparams, hipStream_t stream) {
     void* args[] = {(void*)&dparams};
-    const void* fn = nullptr;
-    int bytes = rc_trajectory_smem_bytes(nb, split);
-    switch (nb * 2 + (split ? 1 : 0)) {
-        case 2: fn = (const void*)rc_trajectory<1, false>; break;
-        case 3: fn = (const void*)rc_trajectory<1, true>; break;
-        case 4: fn = (const void*)rc_trajectory<2, false>; break;
-        case 5: fn = (const void*)rc_trajectory<2, true>; break;
-        case 6: fn = (const void*)rc_trajectory<3, false>; break;
-        case 7: fn = (const void*)rc_trajectory<3, true>; break;
-        case 8: fn = (const void*)rc_trajectory<4, false>; break;
-        case 9: fn = (const void*)rc_trajectory<4, true>; break;
-        default: return hipErrorInvalidValue;
-    }
+    const void* fn = trajectory_fn(nb, split);
+    const int bytes = rc_trajectory_smem_bytes(nb, split);
+    if (!fn) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
     return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(256), args, (unsigned)bytes, stream);

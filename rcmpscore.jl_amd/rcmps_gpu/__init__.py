@@ -5,7 +5,8 @@ continuous MPS), its state preparation and optimizer, running on hand-written HI
 from ._lib import DeviceControlFailure, SolverFailure
 from .autograd import Phi4Energy, lbfgs_rk, phi4_energy
 from .io import StateRecord, rcmps_load, rcmps_store, read_portable, write_portable
-from .observables import boundary_environment, fg_phi4, fg_phi4_raw, fg_phi4_rk, trajectory_samples
+from .observables import (boundary_environment, fg_phi4, fg_phi4_batch, fg_phi4_raw, fg_phi4_rk,
+                          trajectory_samples)
 from .optimization import (BackwardLineSearch, metric, optimize_state, precondition, retraction)
 from .state import (IntegratorParameters, ManifoldParameters, ModelParameters, RandomState, RandomStateZ2,
                     State, random_state_arrays, random_state_z2_arrays)
@@ -17,5 +18,5 @@ __all__ = [
     "ModelParameters", "RandomState", "RandomStateZ2", "SolverFailure", "State", "boundary_environment",
     "fg_phi4", "fg_phi4_raw", "metric", "optimize_state", "precondition", "random_state_arrays",
     "random_state_z2_arrays", "retraction", "trajectory_samples", "StateRecord", "rcmps_load", "rcmps_store",
-    "read_portable", "write_portable", "Phi4Energy", "fg_phi4_rk", "lbfgs_rk", "phi4_energy",
+    "read_portable", "write_portable", "Phi4Energy", "fg_phi4_batch", "fg_phi4_rk", "lbfgs_rk", "phi4_energy",
 ]

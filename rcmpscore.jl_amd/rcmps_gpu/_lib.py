@@ -84,6 +84,9 @@ SIGNATURES = {
     "rcmps_fg_phi4": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(Integrator), ctypes.c_double,
                                      ctypes.c_int, ctypes.POINTER(ctypes.c_double), _P, _P, _P,
                                      ctypes.POINTER(Stats)]),
+    "rcmps_fg_phi4_batch": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(Integrator), ctypes.c_double, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_P), ctypes.POINTER(Stats)]),
     "rcmps_fg_phi4_rk": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(Integrator), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double), _P, _P, ctypes.POINTER(Stats)]),
     "rcmps_boundary_environment": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Stats)]),

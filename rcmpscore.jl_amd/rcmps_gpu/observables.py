@@ -75,6 +75,41 @@ def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str =
     return (energy, grad, stats) if return_stats else (energy, grad)
 
 
+def fg_phi4_batch(states, couplings, *, gradient: bool = True, precision: str = "fp64",
+                  integrator: IntegratorParameters | None = None, metric_reg: float | None = None,
+                  return_stats: bool = False):
+    """fg_phi4 of many independent states (any mix of D and g) in one call: the trajectories of
+    states of one padded size run in one persistent launch.  Returns (energies, gradients) with
+    gradients as numpy arrays (None when gradient=False), plus the per-state stats dicts when
+    return_stats.  Raises SolverFailure if any evaluation failed."""
+    states = list(states)
+    gs = [float(x) for x in couplings]
+    if not states or len(gs) != len(states):
+        raise ValueError("one coupling per state is required")
+    device = states[0].device
+    if any(s.device != device for s in states):
+        raise ValueError("all states must live on one device")
+    lib = _lib.load()
+    ctx = _lib.context(device)
+    ip = _integrator(integrator or states[0].integrator_param)
+    reg = states[0].manifold_param.metric_reg if metric_reg is None else metric_reg
+    dev = torch.device("cuda", device)
+    n = len(states)
+    grads = [torch.empty((s.D, s.D), dtype=torch.complex128, device=dev) for s in states] if gradient else None
+    handles = (ctypes.c_void_p * n)(*[s.handle.value for s in states])
+    garr = (ctypes.c_void_p * n)(*[g.data_ptr() for g in grads]) if gradient else None
+    energies = (ctypes.c_double * n)()
+    stats = (_lib.Stats * n)()
+    st = lib.rcmps_fg_phi4_batch(ctx, n, handles, (ctypes.c_double * n)(*gs), ctypes.byref(ip), float(reg),
+                                 _flags(True, precision), energies, garr, stats)
+    _lib.check(st, "fg_phi4_batch", stats[0] if st == 0 else next((x for x in stats if x.status), stats[0]))
+    out_e = list(energies)
+    out_g = [g.cpu().numpy() for g in grads] if gradient else None
+    if return_stats:
+        return out_e, out_g, [x.as_dict() for x in stats]
+    return out_e, out_g
+
+
 def fg_phi4_rk(s: State, g: float, *, gradient: bool = True, boundary: bool = True,
                integrator: IntegratorParameters | None = None, precision: str = "fp64"):
     """(energy, grad_R, grad_K, stats): the energy of State(R, K) and its Euclidean gradient in
