@@ -4,7 +4,11 @@ Workload: the D=64 phi^4 state of the headline metric: R, K complex normal scale
 Hermitised, seed 42, numpy PCG64 draws in RandomState order), g = 2.0, n_quad = 256 (512 Gauss
 nodes), max_point = 20, atol = 1e-11, rtol = 1e-10.  A step is one full fg_phi4 evaluation
 (six DOP853 trajectories, pullback, boundary Poisson solve, metric solve) on a prepared state
-whose inputs are resident in HBM.  Prints one JSON line.
+whose inputs are resident in HBM.  `value` is the split-precision path (trajectory products on
+the INT8 MFMA over exact 8-bit slices); the plain FP64 path is timed the same way beside it, and
+the split result's energy and gradient errors against it are reported.  --batch N instead times
+the batched workload (N states, D uniform in {8, 16, 24, 32, 48, 64}, g uniform in [0.5, 4]).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -23,6 +27,8 @@ for p in (str(ROOT), str(ROOT / "rcmpscore.jl_amd")):
         sys.path.insert(0, p)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
+INT8_MFMA_PEAK_POPS = 5.0
+SPLIT_PAIRS = 26   # digit products per real product in the split kernel (6 digits, diagonals 0..6)
 METRIC = "fg_phi4 E+grad evals/s at D=64, split-prec vs FP64 on GPU at |ΔE|/|E|≤1e-10"
 # complex D x D products per right-hand side of each reduced system (level-1 and level-2 tiles):
 # kinetic fwd, kinetic adj, kinetic-commutator fwd, kinetic-commutator adj, phi4 fwd, phi4 adj
@@ -79,6 +85,62 @@ def cpu_baseline(R, Q, rho, wl, gpu_stats, attempts_sample=10):
                 estimated_seconds_per_eval=est)
 
 
+def batch_main(args):
+    """The batched workload: args.batch states, D uniform in {8, 16, 24, 32, 48, 64}, g uniform in
+    [0.5, 4], R, K complex normal scaled 0.2 seeded 0..N-1, default integrator; a step is one
+    rcmps_fg_phi4_batch over all states (energies and Riemannian gradients)."""
+    import numpy as np
+    import torch
+
+    import rcmps_gpu
+    rng = np.random.default_rng(0)
+    dims = rng.choice([8, 16, 24, 32, 48, 64], size=args.batch)
+    gs = rng.uniform(0.5, 4.0, size=args.batch)
+    t0 = time.perf_counter()
+    states = []
+    for i, d in enumerate(dims):
+        r, k = rcmps_gpu.random_state_arrays(int(d), scaling=0.2, seed=i)
+        states.append(rcmps_gpu.State(r, k))
+    prep_s = time.perf_counter() - t0
+    result = {}
+    for precision in ("split", "fp64"):
+        for _ in range(args.warmup):
+            rcmps_gpu.fg_phi4_batch(states, gs, precision=precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            E, G, stats = rcmps_gpu.fg_phi4_batch(states, gs, precision=precision, return_stats=True)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        result[precision] = dict(elapsed=elapsed, E=np.array(E), G=G,
+                                 launch_ms=sorted({round(x["trajectory_ms"], 2) for x in stats}))
+    sp, fp = result["split"], result["fp64"]
+    n = args.batch
+    out = {
+        "metric": "batched fg_phi4 E+grad evals/s (uneven D and g, one persistent launch per padded size)",
+        "value": n * args.steps / sp["elapsed"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * sp["elapsed"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (split: int8 MFMA over exact digit slices)",
+        "data": "synthetic",
+        "config": {"workload": f"{n} states, D uniform in {{8,16,24,32,48,64}}, g uniform in [0.5,4], "
+                               "scaling 0.2, seeds 0..N-1, n_quad=128, max_point=20, atol=1e-11, rtol=1e-10",
+                   "precision": "split", "states": n, "dims": {str(d): int((dims == d).sum()) for d in sorted(set(dims))}},
+        "fp64": {"value": n * args.steps / fp["elapsed"], "ms_per_step": 1e3 * fp["elapsed"] / args.steps,
+                 "launch_ms": fp["launch_ms"]},
+        "split_launch_ms": sp["launch_ms"],
+        "split_vs_fp64": fp["elapsed"] / sp["elapsed"],
+        "split_error_vs_fp64": {"energy_rel": float(np.max(np.abs(sp["E"] - fp["E"]) / np.abs(fp["E"]))),
+                                "gradient_rel": float(max(np.abs(a - b).max() / np.abs(b).max()
+                                                          for a, b in zip(sp["G"], fp["G"])))},
+        "state_prep_s": prep_s,
+    }
+    print(json.dumps(out), flush=True)
+    for st in states:
+        st.close()
+    torch.cuda.synchronize()
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,11 +153,15 @@ def main(argv=None):
     ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of this many MiB (a known-size kernel that "
                          "calibrates HBM byte counters in a profile of the same command)")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="time the batched workload of N states instead of the single D=64 evaluation")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent states shard across GPUs with no exchange)", file=sys.stderr)
         return 2
+    if args.batch > 0:
+        return batch_main(args)
     import numpy as np
     import torch
 
@@ -115,39 +181,62 @@ def main(argv=None):
     prep_s = time.perf_counter() - t0
     ip = s.integrator_param
     ip.n_quad, ip.max_point, ip.atol, ip.rtol = wl["n_quad"], wl["max_point"], wl["atol"], wl["rtol"]
-    for _ in range(args.warmup):
-        rcmps_gpu.fg_phi4_raw(s, wl["g"])
-    torch.cuda.synchronize()
-    traj_ms, energies, stats = [], [], None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e, G, _, _, stats = rcmps_gpu.fg_phi4_raw(s, wl["g"])
-        traj_ms.append(stats["trajectory_ms"])
-        energies.append(e)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    value = args.steps / elapsed
     D = wl["D"]
-    flops = sum(stats["rhs_calls"][i] * GEMMS_PER_RHS[i] for i in range(6)) * 8.0 * D**3
-    avg_traj_s = sum(traj_ms) / len(traj_ms) / 1e3
-    achieved = flops / avg_traj_s / 1e12
+
+    def timed(precision):
+        for _ in range(args.warmup):
+            rcmps_gpu.fg_phi4_raw(s, wl["g"], precision=precision)
+        torch.cuda.synchronize()
+        traj_ms, energies, stats, G = [], [], None, None
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            e, G, _, _, stats = rcmps_gpu.fg_phi4_raw(s, wl["g"], precision=precision)
+            traj_ms.append(stats["trajectory_ms"])
+            energies.append(e)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        flops = sum(stats["rhs_calls"][i] * GEMMS_PER_RHS[i] for i in range(6)) * 8.0 * D**3
+        avg_traj_s = sum(traj_ms) / len(traj_ms) / 1e3
+        return dict(value=args.steps / elapsed, elapsed=elapsed, energies=energies, G=G.cpu().numpy(),
+                    stats=stats, traj_s=avg_traj_s, flops=flops)
+
+    split = timed("split")       # the metric's path: split-precision trajectory products
+    fp64 = timed("fp64")
+    stats = split["stats"]
+    achieved = split["flops"] / split["traj_s"] / 1e12
+    # INT8 MFMA work actually issued: per FP64-equivalent real multiply-add, RC_SPLIT_PAIRS digit
+    # products of k = 64 (the split kernel's digit pairs with d + e <= 6 out of 6 x 6)
+    int8_pops = split["flops"] / 2 * SPLIT_PAIRS * 2 / split["traj_s"] / 1e15
     roofline = dict(bound="mfma", achieved=achieved, peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=achieved / FP64_MFMA_PEAK_TFLOPS, traffic=profiled_traffic(wl),
-                    kernel="rc_trajectory<4>", kernel_ms=avg_traj_s * 1e3,
-                    algorithmic_tflop_per_launch=flops / 1e12)
+                    kernel="rc_trajectory<4, split>", kernel_ms=split["traj_s"] * 1e3,
+                    algorithmic_tflop_per_launch=split["flops"] / 1e12,
+                    note=("achieved = FP64-equivalent algorithmic TFLOP/s of the D x D products against the "
+                          "FP64 MFMA peak; int8_mfma_pops is the INT8 MFMA work issued for them against "
+                          f"{INT8_MFMA_PEAK_POPS} POP/s"),
+                    int8_mfma_pops=int8_pops, int8_frac=int8_pops / INT8_MFMA_PEAK_POPS)
+    fp64_achieved = fp64["flops"] / fp64["traj_s"] / 1e12
+    e64, esp = fp64["energies"][-1], split["energies"][-1]
     out = {
-        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "metric": METRIC, "value": split["value"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * split["elapsed"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (split: int8 MFMA over exact digit slices)",
+        "data": "synthetic",
         "config": {"workload": f"fg_phi4 D={D} g={wl['g']} seeded random state (scaling {wl['scaling']}, "
                                f"seed {wl['seed']}), n_quad={wl['n_quad']}, max_point={wl['max_point']}, "
                                f"atol={wl['atol']}, rtol={wl['rtol']}",
-                   "precision": "fp64", "D": D, "n_quad": wl["n_quad"], "g": wl["g"]},
+                   "precision": "split", "D": D, "n_quad": wl["n_quad"], "g": wl["g"]},
         "roofline": roofline,
-        "energy": energies[-1],
-        "energy_spread": max(energies) - min(energies),
+        "fp64": {"value": fp64["value"], "ms_per_step": 1e3 * fp64["elapsed"] / args.steps,
+                 "trajectory_ms": fp64["traj_s"] * 1e3, "energy": e64,
+                 "roofline_achieved_tflops": fp64_achieved, "roofline_frac": fp64_achieved / FP64_MFMA_PEAK_TFLOPS},
+        "split_vs_fp64": split["value"] / fp64["value"],
+        "split_error_vs_fp64": {"energy_rel": abs(esp - e64) / abs(e64),
+                                "gradient_rel": float(np.abs(split["G"] - fp64["G"]).max() / np.abs(fp64["G"]).max())},
+        "energy": esp,
+        "energy_spread": max(split["energies"]) - min(split["energies"]),
         "attempts": stats["attempts"], "dense_steps": stats["dense_steps"],
-        "trajectory_ms": avg_traj_s * 1e3, "pullback_ms": stats["pullback_ms"],
+        "trajectory_ms": split["traj_s"] * 1e3, "pullback_ms": stats["pullback_ms"],
         "boundary_ms": stats["boundary_ms"], "state_prep_s": prep_s,
     }
     if not args.no_cpu_baseline:
