@@ -44,6 +44,10 @@ extern "C" {
    recombination); integration state, error control, dense output, pullback, boundary and
    metric solves stay FP64. */
 #define RCMPS_FLAG_SPLIT 2
+/* kinetic_backend=:krylov (src/observables.jl:229-236): both kinetic terms on the infinite line by
+   the angular resolvent with matrix-free Arnoldi (the context's rcmps_kinetic_options); only the
+   :phi^4: trajectories run. */
+#define RCMPS_FLAG_KINETIC_KRYLOV 4
 
 typedef struct rcmps_context rcmps_context;
 typedef struct rcmps_state rcmps_state;
@@ -57,6 +61,15 @@ typedef struct {
     long long maxiters;
     double initial_step;
 } rcmps_integrator;
+
+/* KineticOptions (src/kinetic.jl:9-16) and KineticKrylovOptions (src/kinetic_krylov.jl:8-13). */
+typedef struct {
+    int minnodes, maxnodes;        /* angular Gauss nodes, doubled from minnodes up to maxnodes */
+    double atol, rtol;             /* convergence of energy and finished gradient between doublings */
+    double tolerance;              /* true relative residual of every shifted solve */
+    double fallback_tolerance;     /* residual accepted at basis exhaustion or maxdim (>= tolerance) */
+    int initialdim, maxdim;        /* Arnoldi basis sizes */
+} rcmps_kinetic_options;
 
 typedef struct {
     int status;
@@ -72,6 +85,12 @@ typedef struct {
        step end (error norm, record, all-gather), whole kernel, publishing the next stage input,
        stage entry */
     double phase_us[6][12];
+    /* infinite-line kinetic backend: angular nodes of the accepted level, basis sizes per
+       (op R/[Q,R]) x (forward/adjoint), operator applications, largest true relative residual */
+    int kinetic_nodes;
+    int kinetic_basis[4];
+    int kinetic_actions;
+    double kinetic_residual;
 } rcmps_stats;
 
 int rcmps_context_create(int device, rcmps_context** out);
@@ -116,6 +135,18 @@ int rcmps_fg_phi4_batch(rcmps_context* ctx, int n, const rcmps_state* const* sta
    computed and only the three forward trajectories run. */
 int rcmps_fg_phi4_rk(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip, int flags,
                      double* energy, double* grad_R, double* grad_K, rcmps_stats* stats);
+
+/* Set the infinite-line kinetic controls of this context (defaults: 32, 512, 1e-11, 1e-10, 1e-12,
+   1e-12, 16, 512).  RCMPS_ERR_ARGUMENT for invalid controls (ArgumentError in the reference). */
+int rcmps_set_kinetic_options(rcmps_context* ctx, const rcmps_kinetic_options* opt);
+
+/* kinetic1 (which = 1), kinetic2 (which = 2) or both (which = 3) on the infinite line by the Krylov
+   backend: kinetic_resolvent (src/kinetic.jl:77-124) over kinetic_components
+   (src/kinetic_krylov.jl:156-189).  raw_cov, raw_env (device, may be null) are the covector and
+   environment before finish_gradient; grad (device, may be null) the finished Riemannian gradient
+   (metric_reg as in rcmps_fg_phi4). */
+int rcmps_kinetic(rcmps_context* ctx, const rcmps_state* st, int which, double metric_reg, double* value,
+                  double* raw_cov, double* raw_env, double* grad, rcmps_stats* stats);
 
 /* Solve (-L* + |I><rho0|) H = a - tr(rho0 a) I, tr(rho0 H) = 0. */
 int rcmps_boundary_environment(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol,

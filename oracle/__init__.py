@@ -52,6 +52,9 @@ def lib():
         _lib.or_fg_phi4_rk.argtypes = [ctypes.c_int, p, p, p, ctypes.c_double, ctypes.POINTER(Integrator),
                                        ctypes.c_double, ctypes.c_int, ctypes.c_int, p, p, p]
         _lib.or_trajectory.argtypes = [ctypes.c_int, p, p, p, ctypes.c_int, ctypes.POINTER(Integrator), p, p]
+        _lib.or_kinetic_resolvent.argtypes = [ctypes.c_int, p, p, p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                              p, p, p, p]
         _lib.or_system_len.restype = ctypes.c_size_t
         _lib.or_system_len.argtypes = [ctypes.c_int, ctypes.c_int]
         _lib.or_steady_state.argtypes = [ctypes.c_int, p, p, ctypes.c_double, ctypes.c_int, p, p]
@@ -101,8 +104,8 @@ def canonical_q(R, K):
 
 def fg_phi4(R, Q, rho, g, *, n_quad=128, max_point=20.0, atol=1e-11, rtol=1e-10, initial_step=0.01,
             max_steps=1_000_000, metric_reg=1e-3, boundary_tol=1e-13, boundary=True,
-            boundary_method="iterative", nthreads=6):
-    """E, G and diagnostics of the oracle's fg_phi4."""
+            boundary_method="iterative", nthreads=6, kinetic_backend="ode"):
+    """E, G and diagnostics of the oracle's fg_phi4 (kinetic_backend "ode" or "resolvent")."""
     d = len(R)
     R, pr = _c(R)
     Q, pq = _c(Q)
@@ -114,7 +117,7 @@ def fg_phi4(R, Q, rho, g, *, n_quad=128, max_point=20.0, atol=1e-11, rtol=1e-10,
     env = np.zeros((d, d), np.complex128)
     terms = np.zeros(3)
     stats = np.zeros(30, dtype=np.int64)
-    flags = (0 if boundary else 1) | (2 if boundary_method == "dense" else 0)
+    flags = (0 if boundary else 1) | (2 if boundary_method == "dense" else 0) | (4 if kinetic_backend == "resolvent" else 0)
     st = lib().or_fg_phi4(d, pr, pq, prho, float(g), ctypes.byref(ip), float(metric_reg),
                           float(boundary_tol), flags, int(nthreads), ctypes.addressof(e),
                           G.ctypes.data, cov.ctypes.data, env.ctypes.data, terms.ctypes.data,
@@ -145,6 +148,42 @@ def fg_phi4_rk(R, K, g, *, rho=None, n_quad=128, max_point=20.0, atol=1e-11, rto
                              int(nthreads), ctypes.addressof(e), GR.ctypes.data, GK.ctypes.data)
     _check(st, "oracle fg_phi4_rk")
     return e.value, GR, GK
+
+
+def kinetic_resolvent(R, Q, rho, *, minnodes=32, maxnodes=512, atol=1e-11, rtol=1e-10, metric_reg=1e-3,
+                      boundary_tol=1e-13, gradient=True):
+    """kinetic1 + kinetic2 on the infinite line by the dense-Schur angular resolvent with node
+    doubling: (value, covector, environment, nodes used)."""
+    d = len(R)
+    R, pr = _c(R)
+    Q, pq = _c(Q)
+    rho, prho = _c(rho)
+    v = ctypes.c_double()
+    used = ctypes.c_int()
+    cov = np.zeros((d, d), np.complex128)
+    env = np.zeros((d, d), np.complex128)
+    st = lib().or_kinetic_resolvent(d, pr, pq, prho, int(minnodes), int(maxnodes), float(atol), float(rtol),
+                                    float(metric_reg), float(boundary_tol), 1 if gradient else 0,
+                                    ctypes.addressof(v), cov.ctypes.data, env.ctypes.data, ctypes.addressof(used))
+    _check(st, "oracle kinetic_resolvent")
+    return v.value, cov, env, used.value
+
+
+def metric_solve_full(R, Q, rho, covector, environment, metric_reg=1e-3, boundary_tol=1e-13):
+    """finish_gradient (src/observables.jl:201-212) of a raw covector/environment pair."""
+    R = np.asarray(R, np.complex128)
+    h = boundary_environment(Q, R, rho, environment, tol=boundary_tol, method="dense")
+    c = np.asarray(covector, np.complex128) + rho @ R.conj().T @ h - rho @ h @ R.conj().T
+    return _metric_solve(rho, metric_reg, c)
+
+
+def _metric_solve(rho, reg, cov):
+    d = len(rho)
+    rho, prho = _c(rho)
+    cov, pc = _c(cov)
+    G = np.zeros((d, d), np.complex128)
+    _check(lib().or_metric_solve(d, prho, float(reg), pc, G.ctypes.data), "oracle metric_solve")
+    return G
 
 
 def system_len(d: int, system: int) -> int:

@@ -1203,6 +1203,10 @@ void or_rhs_full(int n, const cx* q, const cx* r, int term, int adj, double t, c
  * per-term values, per-system stats (6 x 5 longs).  flags bit0: skip the boundary term;
  * bit1: dense boundary solve.  Returns a status code.
  */
+int or_kinetic_resolvent(int n, const cx* r, const cx* q, const cx* rho, int minnodes, int maxnodes, double atol,
+                         double rtol, double metric_reg, double boundary_tol, int gradient, double* value, cx* cov,
+                         cx* env, int* nodes_used);
+
 static int fg_phi4_core(int n, const cx* r, const cx* q, const cx* rho, double g, const or_integrator_t* ip,
                         double metric_reg, double boundary_tol, int flags, int nthreads,
                         double* energy, cx* grad, cx* raw_cov, cx* raw_env, double* term_values, long* stats,
@@ -1210,6 +1214,9 @@ static int fg_phi4_core(int n, const cx* r, const cx* q, const cx* rho, double g
     size_t m = (size_t)n * n;
     if (n < 1 || ip->n_quad < 8 || !(ip->max_point > 0) || !(ip->atol > 0) || !(ip->rtol > 0))
         return OR_BAD_ARGUMENT;
+    /* flags bit2: kinetic terms on the infinite line (kinetic_backend=:resolvent, default options) */
+    const int kin_res = (flags & 4) != 0;
+    if (kin_res && grad_r) return OR_BAD_ARGUMENT;
     cx* comm = cx_alloc(m);
     mm(n, 1, q, 0, r, 0, 0, comm);
     mm(n, -1, r, 0, q, 0, 1, comm);
@@ -1227,8 +1234,14 @@ static int fg_phi4_core(int n, const cx* r, const cx* q, const cx* rho, double g
     }
     long st_local[30];
 #pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 1)
-    for (int s = 0; s < 6; ++s)
-        status[s] = run_system(&sys[s], ip, pts, nn + 1, samples[s], st_local + 5 * s);
+    for (int s = 0; s < 6; ++s) {
+        if (kin_res && s < 4) {
+            status[s] = OR_OK;
+            memset(st_local + 5 * s, 0, 5 * sizeof(long));
+        } else {
+            status[s] = run_system(&sys[s], ip, pts, nn + 1, samples[s], st_local + 5 * s);
+        }
+    }
     if (stats) memcpy(stats, st_local, sizeof st_local);
     int st = OR_OK;
     for (int s = 0; s < 6 && !st; ++s) st = status[s];
@@ -1240,8 +1253,17 @@ static int fg_phi4_core(int n, const cx* r, const cx* q, const cx* rho, double g
     cx* e1 = cx_alloc(m);
     cx *xr = NULL, *xq = NULL, *xr1 = NULL, *xq1 = NULL;
     if (grad_r) { xr = cx_alloc(m); xq = cx_alloc(m); xr1 = cx_alloc(m); xq1 = cx_alloc(m); }
+    if (!st && kin_res) {   /* fixed reduction order: the kinetic job first (src/observables.jl:236-246) */
+        double kv;
+        st = or_kinetic_resolvent(n, r, q, rho, 32, 512, 1e-11, 1e-10, metric_reg, boundary_tol, 1, &kv, c1, e1, NULL);
+        if (!st) {
+            value += kv;
+            for (size_t i = 0; i < m; ++i) { cov[i] += c1[i]; env[i] += e1[i]; }
+            if (term_values) { term_values[0] = kv; term_values[1] = 0; }
+        }
+    }
     if (!st) {
-        for (int term = 0; term < 3; ++term) {
+        for (int term = kin_res ? 2 : 0; term < 3; ++term) {
             double v;
             pullback(&sys[2 * term], &sys[2 * term + 1], samples[2 * term], samples[2 * term + 1],
                      pts, wts, nn, c1, e1, &v, xr1, xq1);
@@ -1310,6 +1332,190 @@ int or_fg_phi4_rk(int n, const cx* r, const cx* q, const cx* rho, double g, cons
     if (!grad_r || !grad_k) return OR_BAD_ARGUMENT;
     return fg_phi4_core(n, r, q, rho, g, ip, 0.0, boundary_tol, flags, nthreads, energy, NULL, NULL, NULL, NULL,
                         NULL, grad_r, grad_k);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* infinite-line kinetic terms by the angular resolvent (src/kinetic.jl:1-124), dense Schur     */
+/* ------------------------------------------------------------------------------------------ */
+
+/* transfer matrix in row-major vec: vec(L(X)) = T vec(X)  (src/transfer.jl:21-26 up to the vec order) */
+static void transfer_matrix(int n, const cx* q, const cx* r, cx* T) {
+    size_t m = (size_t)n * n;
+    cx* e = cx_alloc(m);
+    cx* col = cx_alloc(m);
+    for (size_t j = 0; j < m; ++j) {
+        memset(e, 0, sizeof(cx) * m);
+        e[j] = 1;
+        lindblad(n, q, r, e, col, 0);
+        for (size_t i = 0; i < m; ++i) T[i * m + j] = col[i];
+    }
+    free(e); free(col);
+}
+
+/* Gauss-Legendre nodes on (0, pi/2) with weights summing to one (the factor 2/pi included),
+   src/kinetic.jl:19-23 (there by Golub-Welsch) */
+static void kinetic_grid(int nodes, double* theta, double* w) {
+    gauss_legendre(nodes, theta, w);
+    for (int i = 0; i < nodes; ++i) {
+        theta[i] = M_PI / 4 * (theta[i] + 1);
+        w[i] /= 2;
+    }
+}
+
+/* One kinetic spec: (2/pi) int_0^{pi/2} tr[op (1 - cos(theta) L)^{-1} (rho op^+)] dtheta, with its
+   covector and environment (src/kinetic.jl:25-60).  U, Z: Schur factors of T (T = Z U Z^+). */
+static void kinetic_components(int n, const cx* q, const cx* r, const cx* rho, int commutator, const cx* U,
+                               const cx* Z, int nodes, const double* theta, const double* wts, int gradient,
+                               double* value, cx* cov, cx* env) {
+    size_t m = (size_t)n * n;
+    cx *op = cx_alloc(m), *opd = cx_alloc(m), *rhoopd = cx_alloc(m), *b = cx_alloc(m), *c = cx_alloc(m);
+    cx *xx = cx_alloc(m), *yy = cx_alloc(m), *x = cx_alloc(m), *A = cx_alloc(m), *a = cx_alloc(m);
+    cx *M = cx_alloc(m), *N = cx_alloc(m), *u = cx_alloc(m), *v = cx_alloc(m), *E = cx_alloc(m), *tmp = cx_alloc(m);
+    if (commutator) { mm(n, 1, q, 0, r, 0, 0, op); mm(n, -1, r, 0, q, 0, 1, op); }
+    else memcpy(op, r, sizeof(cx) * m);
+    adjoint_of(n, op, opd);
+    mm(n, 1, rho, 0, opd, 0, 0, rhoopd);
+    for (size_t i = 0; i < m; ++i) {      /* b = Z^+ vec(rho op^+), c = Z^+ vec(op^+) */
+        cx sb = 0, sc = 0;
+        for (size_t k = 0; k < m; ++k) { sb += conj(Z[k * m + i]) * rhoopd[k]; sc += conj(Z[k * m + i]) * opd[k]; }
+        b[i] = sb; c[i] = sc;
+    }
+    double val = 0;
+    for (int t = 0; t < nodes; ++t) {
+        const double z = cos(theta[t]), w = wts[t];
+        /* (I - z U) xx = b, upper triangular */
+        for (long i = (long)m - 1; i >= 0; --i) {
+            cx sacc = b[i];
+            for (size_t k = (size_t)i + 1; k < m; ++k) sacc += z * U[(size_t)i * m + k] * xx[k];
+            xx[i] = sacc / (1.0 - z * U[(size_t)i * m + i]);
+        }
+        cx d = 0;
+        for (size_t i = 0; i < m; ++i) d += conj(c[i]) * xx[i];
+        val += w * creal(d);
+        if (!gradient) continue;
+        /* (I - z U)^+ yy = c, lower triangular */
+        for (size_t i = 0; i < m; ++i) {
+            cx sacc = c[i];
+            for (size_t k = 0; k < i; ++k) sacc += z * conj(U[k * m + i]) * yy[k];
+            yy[i] = sacc / (1.0 - z * conj(U[i * m + i]));
+        }
+        for (size_t i = 0; i < m; ++i) {
+            cx sx = 0, sa = 0;
+            for (size_t k = 0; k < m; ++k) { sx += Z[i * m + k] * xx[k]; sa += Z[i * m + k] * yy[k]; }
+            x[i] = sx; A[i] = sa;
+        }
+        adjoint_of(n, A, a);
+        addtriple(n, M, x, 0, r, 1, a, 0, w * z, tmp);
+        addtriple(n, M, x, 0, a, 0, r, 1, -w * z, tmp);
+        addtriple(n, N, a, 0, r, 0, x, 0, w * z, tmp);
+        addtriple(n, N, r, 0, a, 0, x, 0, -w * z, tmp);
+        for (size_t i = 0; i < m; ++i) u[i] += w * x[i];
+        mm(n, w, a, 0, rho, 0, 1, v);
+        mm(n, w, opd, 0, a, 0, 1, E);
+    }
+    *value = val;
+    if (gradient) {
+        if (commutator) {
+            mm(n, 1, u, 0, q, 0, 1, M); mm(n, -1, q, 0, u, 0, 1, M);
+            addtriple(n, M, r, 0, u, 0, r, 1, -1, tmp);
+            addtriple(n, M, u, 0, r, 0, r, 1, 1, tmp);
+            mm(n, 1, q, 1, v, 0, 1, N); mm(n, -1, v, 0, q, 1, 1, N);
+            addtriple(n, N, r, 0, v, 0, r, 1, -1, tmp);
+            addtriple(n, N, r, 0, r, 1, v, 0, 1, tmp);
+        } else {
+            for (size_t i = 0; i < m; ++i) { M[i] += u[i]; N[i] += v[i]; }
+        }
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                size_t ik = (size_t)i * n + k, ki = (size_t)k * n + i;
+                cov[ik] = (M[ik] + conj(N[ki])) / 2;
+                env[ik] = (E[ik] + conj(E[ki])) / 2;
+            }
+    }
+    free(op); free(opd); free(rhoopd); free(b); free(c); free(xx); free(yy); free(x); free(A); free(a);
+    free(M); free(N); free(u); free(v); free(E); free(tmp);
+}
+
+/* kinetic1 + kinetic2 at a fixed node count (src/kinetic.jl:62-75 with weights (1, 1)) */
+static void kinetic_sum(int n, const cx* q, const cx* r, const cx* rho, const cx* U, const cx* Z, int nodes,
+                        int gradient, double* value, cx* cov, cx* env) {
+    size_t m = (size_t)n * n;
+    double* th = (double*)malloc(sizeof(double) * (size_t)nodes);
+    double* w = (double*)malloc(sizeof(double) * (size_t)nodes);
+    kinetic_grid(nodes, th, w);
+    cx *c1 = cx_alloc(m), *e1 = cx_alloc(m);
+    *value = 0;
+    if (gradient) { memset(cov, 0, sizeof(cx) * m); memset(env, 0, sizeof(cx) * m); }
+    for (int comm = 0; comm < 2; ++comm) {
+        double v;
+        kinetic_components(n, q, r, rho, comm, U, Z, nodes, th, w, gradient, &v, c1, e1);
+        *value += v;
+        if (gradient)
+            for (size_t i = 0; i < m; ++i) { cov[i] += c1[i]; env[i] += e1[i]; }
+    }
+    free(th); free(w); free(c1); free(e1);
+}
+
+/* finish_gradient (src/observables.jl:201-212) of a covector/environment pair */
+static int finish_gradient(int n, const cx* q, const cx* r, const cx* rho, const cx* cov, const cx* env,
+                           double metric_reg, double boundary_tol, cx* grad) {
+    size_t m = (size_t)n * n;
+    cx *c = cx_alloc(m), *h = cx_alloc(m), *tmp = cx_alloc(m);
+    memcpy(c, cov, sizeof(cx) * m);
+    int st = or_boundary(n, q, r, rho, env, boundary_tol, 1, h);
+    if (!st) {
+        addtriple(n, c, rho, 0, r, 1, h, 0, 1, tmp);
+        addtriple(n, c, rho, 0, h, 0, r, 1, -1, tmp);
+        st = metric_solve(n, rho, metric_reg, c, grad);
+    }
+    free(c); free(h); free(tmp);
+    return st;
+}
+
+/*
+ * kinetic_resolvent (src/kinetic.jl:77-124) for the two kinetic terms: node doubling from minnodes
+ * until the energy and the finished gradient change by at most atol + rtol |.|.  Returns the raw
+ * value, covector and environment of the accepted level and the node count in *nodes_used.
+ */
+int or_kinetic_resolvent(int n, const cx* r, const cx* q, const cx* rho, int minnodes, int maxnodes, double atol,
+                         double rtol, double metric_reg, double boundary_tol, int gradient, double* value, cx* cov,
+                         cx* env, int* nodes_used) {
+    if (n < 1 || minnodes < 2 || maxnodes < 2 * minnodes || !(atol > 0) || !(rtol >= 0)) return OR_BAD_ARGUMENT;
+    size_t m = (size_t)n * n;
+    cx *U = cx_alloc(m * m), *Z = cx_alloc(m * m);
+    transfer_matrix(n, q, r, U);
+    int st = or_schur((int)m, U, Z);
+    if (st) { free(U); free(Z); return st; }
+    cx *pc = cx_alloc(m), *pe = cx_alloc(m), *pg = cx_alloc(m), *g = cx_alloc(m);
+    int nodes = minnodes;
+    double pv;
+    kinetic_sum(n, q, r, rho, U, Z, nodes, gradient, &pv, pc, pe);
+    if (gradient) st = finish_gradient(n, q, r, rho, pc, pe, metric_reg, boundary_tol, pg);
+    int done = 0;
+    while (!st && !done && 2 * nodes <= maxnodes) {
+        nodes *= 2;
+        double v;
+        kinetic_sum(n, q, r, rho, U, Z, nodes, gradient, &v, cov, env);
+        if (!isfinite(v)) { st = OR_NONFINITE; break; }
+        int energy_pass = fabs(v - pv) <= atol + rtol * fabs(v);
+        if (gradient) {
+            st = finish_gradient(n, q, r, rho, cov, env, metric_reg, boundary_tol, g);
+            if (st) break;
+            double dg = 0, gn = 0;
+            for (size_t i = 0; i < m; ++i) { dg += cabs(g[i] - pg[i]) * cabs(g[i] - pg[i]); gn += cabs(g[i]) * cabs(g[i]); }
+            dg = sqrt(dg); gn = sqrt(gn);
+            if (energy_pass && dg <= atol + rtol * gn) done = 1;
+            memcpy(pg, g, sizeof(cx) * m);
+        } else if (energy_pass) {
+            done = 1;
+        }
+        pv = v;
+        *value = v;
+    }
+    if (!st && !done) st = OR_SOLVER_FAILURE;
+    if (nodes_used) *nodes_used = nodes;
+    free(U); free(Z); free(pc); free(pe); free(pg); free(g);
+    return st;
 }
 
 /* Exposed helpers for tests */

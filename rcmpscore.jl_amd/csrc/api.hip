@@ -57,6 +57,28 @@ constexpr int kLive[RC_NSYS] = {1, 2, 1, 2, 3, 4};     // live channels per syst
 constexpr int kHasScalar[RC_NSYS] = {1, 0, 1, 0, 1, 0};
 
 // trajectory workspace of one evaluation (one state) of a launch
+// Arnoldi basis of L or L* on one right-hand side (src/kinetic_krylov.jl:18-33)
+struct KrylovBasis {
+    DevBuf V;                 // (cap + 1) vectors of n = Dp^2 complex
+    std::vector<cd> H;        // (cap + 1) x cap, row-major
+    int n = 0, cap = 0, k = 0;
+    double beta = 0.0;
+    bool exhausted = false, started = false;
+};
+
+// workspace of the infinite-line kinetic backend
+struct KineticWork {
+    KrylovBasis basis[4];     // (op R, op [Q,R]) x (forward, adjoint)
+    DevBuf X, A, LX, T, Yd, zd, wd, wzd, res, hd, tmp;
+    int actions = 0;
+    double max_res = 0.0;
+    void release() {
+        for (auto& b : basis) { b.V.release(); b = KrylovBasis(); }
+        DevBuf* all[] = {&X, &A, &LX, &T, &Yd, &zd, &wd, &wzd, &res, &hd, &tmp};
+        for (auto* d : all) d->release();
+    }
+};
+
 struct TrajBufs {
     DevBuf Y[RC_NSYS], P[RC_NSYS], rec[RC_NSYS], samples[RC_NSYS], E, fixed, flags, out;
     size_t flag_bytes = 0;
@@ -89,8 +111,10 @@ double host_source(double t) {
 }
 
 // Gauss-Legendre on (0, sqrt(max_point)) mirrored (src/observables.jl:65-75): Newton on P_n.
-void quadrature(int n, double max_point, std::vector<double>& pts, std::vector<double>& wts) {
-    std::vector<double> x(n), w(n);
+// Gauss-Legendre nodes and weights on (-1, 1): Newton on P_n from the Chebyshev-like guess.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
     for (int i = 0; i < (n + 1) / 2; ++i) {
         double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0;
         for (int it = 0; it < 100; ++it) {
@@ -116,6 +140,11 @@ void quadrature(int n, double max_point, std::vector<double>& pts, std::vector<d
         x[n - 1 - i] = z;
         w[i] = w[n - 1 - i] = 2.0 / ((1 - z * z) * dp * dp);
     }
+}
+
+void quadrature(int n, double max_point, std::vector<double>& pts, std::vector<double>& wts) {
+    std::vector<double> x, w;
+    gauss_legendre(n, x, w);
     const double a = std::sqrt(max_point);
     pts.assign(2 * n, 0.0);
     wts.assign(2 * n, 0.0);
@@ -150,6 +179,9 @@ struct rcmps_context {
     std::vector<double> q_pts;
     hipEvent_t ev[8];
     double clock_mhz = 100.0;   // constant-rate device clock
+    // infinite-line kinetic backend: controls (KineticOptions / KineticKrylovOptions defaults) and workspace
+    rcmps_kinetic_options kopt{32, 512, 1e-11, 1e-10, 1e-12, 1e-12, 16, 512};
+    KineticWork kin;
 };
 
 struct rcmps_state {
@@ -550,6 +582,7 @@ void rcmps_context_destroy(rcmps_context* ctx) {
                      &ctx->kry_tmp};
     for (auto* b : all) b->release();
     for (auto& b : ctx->tb) b->release();
+    ctx->kin.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -786,7 +819,7 @@ void ensure_quadrature(rcmps_context* ctx, const rcmps_integrator* ip) {
 
 // Fixed insertion sources, the six systems and their tasks for one state (src/observables.jl:7-29:
 // kinetic edges sqrt2 with op R or [Q,R], :phi^4: edges k -> k+1 with coefficient k).
-void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool adjoint, int npts,
+void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool adjoint, bool quartic_only, int npts,
                  std::vector<TrajSys>& sys, std::vector<TrajTask>& tasks) {
     const int D = st->D, Dp = st->Dp, NB = Dp / 16, NT = NB * NB;
     Dev dv{ctx, Dp, D};
@@ -823,7 +856,7 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
         Sy.has_scalar = kHasScalar[s];
         Sy.adj = s % 2;
         Sy.first_task = (int)tasks.size();
-        Sy.ntasks = !adjoint && s % 2 ? 0 : kLive[s] * NT;
+        Sy.ntasks = (!adjoint && s % 2) || (quartic_only && s < 4) ? 0 : kLive[s] * NT;
         Sy.D = D;
         Sy.n_elems = (double)kLive[s] * D * D + kHasScalar[s];
         Sy.Q = st->Q.d(); Sy.Qh = st->Qh.d(); Sy.R = st->R.d(); Sy.Rh = st->Rh.d(); Sy.C = st->C.d();
@@ -883,8 +916,8 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
 
 // Integrate every system of the given evaluations (all of one padded size Dp) in one persistent
 // launch; states are packed into rounds of at most the resident workgroup count.
-void run_class(rcmps_context* ctx, int Dp, bool split, const rcmps_integrator* ip, const std::vector<int>& idx,
-               std::vector<EvalReq>& reqs) {
+void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, const rcmps_integrator* ip,
+               const std::vector<int>& idx, std::vector<EvalReq>& reqs) {
     const int NB = Dp / 16, npts = 2 * ip->n_quad + 1;
     std::vector<TrajSys> sys;
     std::vector<TrajTask> tasks;
@@ -892,7 +925,7 @@ void run_class(rcmps_context* ctx, int Dp, bool split, const rcmps_integrator* i
     std::vector<int> state_first, state_ntasks;
     for (size_t j = 0; j < idx.size(); ++j) {
         const int first = (int)tasks.size();
-        build_state(ctx, *ctx->tb[j], reqs[idx[j]].st, reqs[idx[j]].adjoint(), npts, sys, tasks);
+        build_state(ctx, *ctx->tb[j], reqs[idx[j]].st, reqs[idx[j]].adjoint(), quartic_only, npts, sys, tasks);
         state_first.push_back(first);
         state_ntasks.push_back((int)tasks.size() - first);
     }
@@ -987,10 +1020,27 @@ void run_class(rcmps_context* ctx, int Dp, bool split, const rcmps_integrator* i
     }
 }
 
+// (rho0 + reg I)^{-1} on the device, cached per state and regulariser (observables.jl:210-211)
+const double* metric_inverse(rcmps_context* ctx, const rcmps_state* st, double metric_reg) {
+    rcmps_state* ms = const_cast<rcmps_state*>(st);
+    if (ms->metric_reg_cached != metric_reg) {
+        const int D = st->D;
+        Mat a = st->rho_host;
+        for (int i = 0; i < D; ++i) a[(size_t)i * D + i] += metric_reg;
+        Mat inv;
+        if (!rc::inverse(D, a, inv)) throw Fail{RCMPS_ERR_SOLVER};
+        ck(ms->Minv.ensure((size_t)st->Dp * st->Dp * 16));
+        upload(ctx, D, st->Dp, inv, ms->Minv.d());
+        ms->metric_reg_cached = metric_reg;
+    }
+    return st->Minv.d();
+}
+
 // Pullback over the Gauss nodes, boundary correction and gradients of one evaluation whose
-// trajectories are in tb (src/observables.jl:161-212).
+// trajectories are in tb (src/observables.jl:161-212).  With kin_cov/kin_env (device) the two
+// kinetic terms come from the infinite-line backend and only the :phi^4: trajectories are pulled back.
 void finish_state(rcmps_context* ctx, EvalReq& r, TrajBufs& tb, const rcmps_integrator* ip, double metric_reg,
-                  int flags) {
+                  int flags, const double* kin_cov = nullptr, const double* kin_env = nullptr) {
     const rcmps_state* st = r.st;
     rcmps_stats* S = r.S;
     double *grad = r.grad, *raw_cov = r.raw_cov, *raw_env = r.raw_env, *grad_R = r.grad_R, *grad_K = r.grad_K;
@@ -1024,7 +1074,7 @@ void finish_state(rcmps_context* ctx, EvalReq& r, TrajBufs& tb, const rcmps_inte
     ck(hipMemsetAsync(env, 0, mb, ctx->stream));
     ck(hipMemsetAsync(XR, 0, 2 * mb, ctx->stream));
     const long long node = (long long)Dp * Dp;   // complex elements per matrix
-    for (int term = 0; term < 3; ++term) {
+    for (int term = kin_cov ? 2 : 0; term < 3; ++term) {
         const int sf = 2 * term, sa = sf + 1;
         const int lf = kLive[sf], la = kLive[sa];
         const double* Fs = tb.samples[sf].d();
@@ -1112,9 +1162,11 @@ void finish_state(rcmps_context* ctx, EvalReq& r, TrajBufs& tb, const rcmps_inte
         dv.axpby(cd(1, 0), XQ, cd(c, 0), T2, XQ);
         dv.axpby(cd(1, 0), env, cd(c, 0), tmp, env);
     }
+    if (kin_env) dv.axpby(cd(1, 0), env, cd(1, 0), kin_env, env);
     auto covector = [&] {   // horizontal directions dQ = -R^+ W
         dv.gemm(0, XQ, 0, st->Rh.d(), cterm);
         dv.axpby(cd(1, 0), XR, cd(-1, 0), cterm, cov);
+        if (kin_cov) dv.axpby(cd(1, 0), cov, cd(1, 0), kin_cov, cov);
     };
     covector();
     ck(hipEventRecord(ctx->ev[3], ctx->stream));
@@ -1157,17 +1209,7 @@ void finish_state(rcmps_context* ctx, EvalReq& r, TrajBufs& tb, const rcmps_inte
     }
     if (grad) {
         // G = [(rho + reg I)^{-1} cov]^+ = cov^+ (rho + reg I)^{-1}
-        rcmps_state* ms = const_cast<rcmps_state*>(st);
-        if (ms->metric_reg_cached != metric_reg) {
-            Mat a = st->rho_host;
-            for (int i = 0; i < D; ++i) a[(size_t)i * D + i] += metric_reg;
-            Mat inv;
-            if (!rc::inverse(D, a, inv)) throw Fail{RCMPS_ERR_SOLVER};
-            ck(ms->Minv.ensure(mb));
-            upload(ctx, D, Dp, inv, ms->Minv.d());
-            ms->metric_reg_cached = metric_reg;
-        }
-        dv.gemm(1, cov, 0, st->Minv.d(), tmp);
+        dv.gemm(1, cov, 0, metric_inverse(ctx, st, metric_reg), tmp);
         pad_out(ctx, D, Dp, tmp, grad);
     }
     ck(hipEventRecord(ctx->ev[4], ctx->stream));
@@ -1177,6 +1219,413 @@ void finish_state(rcmps_context* ctx, EvalReq& r, TrajBufs& tb, const rcmps_inte
     ck(hipEventElapsedTime(&bms, ctx->ev[3], ctx->ev[4]));
     S->pullback_ms = pms;
     S->boundary_ms = bms;
+}
+
+// ------------------------------------------------------------------------------------------
+// infinite-line kinetic terms: angular resolvent with matrix-free Arnoldi
+// (src/kinetic.jl:19-124, src/kinetic_krylov.jl:1-189)
+// ------------------------------------------------------------------------------------------
+
+// Gauss-Legendre nodes on (0, pi/2) with weights summing to one (kinetic.jl:19-23)
+void kinetic_grid(int n, std::vector<double>& theta, std::vector<double>& w) {
+    std::vector<double> x;
+    gauss_legendre(n, x, w);
+    theta.resize(n);
+    for (int i = 0; i < n; ++i) {
+        theta[i] = M_PI / 4 * (x[i] + 1);
+        w[i] /= 2;
+    }
+}
+
+// out_b = L(X_b) (or L*(X_b)) for nb Dp x Dp matrices at stride Dp^2; T is nb matrices of scratch
+void lindblad_batched(Dev& dv, const rcmps_state* st, const double* X, double* out, double* T, int nb, bool adj) {
+    const long long node = (long long)dv.Dp * dv.Dp;
+    GemmArgs a{};   // out = Q X (forward), Q^+ X (adjoint)
+    a.A = adj ? st->Qh.d() : st->Q.d(); a.B = X; a.strideB = node; a.C = out; a.strideC = node;
+    dv.batched(a, nb);
+    GemmArgs b{};   // out += X Q^+ (forward), X Q (adjoint)
+    b.A = X; b.strideA = node; b.B = adj ? st->Q.d() : st->Qh.d(); b.C = out; b.strideC = node; b.beta = 1.0;
+    dv.batched(b, nb);
+    GemmArgs c{};   // T = R X (forward), X R (adjoint)
+    if (!adj) { c.A = st->R.d(); c.B = X; c.strideB = node; }
+    else { c.A = X; c.strideA = node; c.B = st->R.d(); }
+    c.C = T; c.strideC = node;
+    dv.batched(c, nb);
+    GemmArgs d{};   // out += T R^+ (forward), R^+ T (adjoint)
+    if (!adj) { d.A = T; d.strideA = node; d.B = st->Rh.d(); }
+    else { d.A = st->Rh.d(); d.B = T; d.strideB = node; }
+    d.C = out; d.strideC = node; d.beta = 1.0;
+    dv.batched(d, nb);
+}
+
+void basis_grow(KrylovBasis& b, int n, int cap, hipStream_t stream) {
+    if (n != b.n) {   // a state of another size: the stored vectors are of no use
+        b.V.release();
+        b.H.clear();
+        b.cap = 0;
+        b.k = 0;
+        b.n = n;
+    }
+    if (cap <= b.cap) return;
+    DevBuf nv;
+    ck(nv.ensure((size_t)(cap + 1) * n * 16));
+    if (b.cap > 0 && b.k + 1 > 0)
+        ck(hipMemcpyAsync(nv.p, b.V.p, (size_t)std::min(b.k + 1, b.cap + 1) * n * 16, hipMemcpyDeviceToDevice, stream));
+    ck(hipStreamSynchronize(stream));
+    b.V.release();
+    b.V = nv;
+    std::vector<cd> H((size_t)(cap + 1) * cap, cd(0, 0));
+    for (int i = 0; i <= b.cap && b.cap > 0; ++i)
+        for (int j = 0; j < b.cap; ++j) H[(size_t)i * cap + j] = b.H[(size_t)i * b.cap + j];
+    b.H.swap(H);
+    b.cap = cap;
+}
+
+// Start a basis on rhs (device, Dp^2 complex): V_0 = rhs / |rhs|
+void basis_start(rcmps_context* ctx, Dev& dv, KrylovBasis& b, const double* rhs, int cap) {
+    b.k = 0;
+    b.exhausted = false;
+    b.started = true;
+    basis_grow(b, dv.n(), cap, ctx->stream);
+    b.beta = dv.norm(rhs);
+    if (!std::isfinite(b.beta)) throw Fail{RCMPS_ERR_SOLVER};
+    if (b.beta == 0.0) { b.exhausted = true; return; }
+    dv.axpby(cd(1.0 / b.beta, 0), rhs, cd(0, 0), nullptr, b.V.d());
+}
+
+// Arnoldi steps up to `target` vectors, classical Gram-Schmidt twice, H from both passes
+// (kinetic_krylov.jl:68-92); exhaustion at a vanishing new direction or the full dimension D^2.
+void basis_extend(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasis& b, bool adj, int target) {
+    KineticWork& kw = ctx->kin;
+    const int n = dv.n();
+    basis_grow(b, n, target, ctx->stream);
+    ck(kw.hd.ensure((size_t)(4 * (b.cap + 1) + 8) * 8));
+    double* V = b.V.d();
+    double* w = ctx->kry_w.d();
+    double* hd0 = kw.hd.d();
+    double* hd1 = hd0 + 2;
+    double* hd2 = hd1 + 2 * (b.cap + 1);
+    double* hnn = hd2 + 2 * (b.cap + 1);
+    std::vector<cd> hv(2 * (b.cap + 1) + 2);
+    const int D2 = st->D * st->D;
+    for (int j = b.k; j < target; ++j) {
+        lindblad(dv, st, V + (size_t)j * n * 2, w, adj, ctx->kry_t1.d());
+        ++kw.actions;
+        hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hd0);
+        hipLaunchKernelGGL(rc_dots, dim3(j + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd1);
+        hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, j + 1, hd1, n, w);
+        hipLaunchKernelGGL(rc_dots, dim3(j + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd2);
+        hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, j + 1, hd2, n, w);
+        hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hnn);
+        ck(hipGetLastError());
+        ck(hipMemcpyAsync(hv.data(), hd0, (size_t)(4 * (b.cap + 1) + 4) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        ck(hipStreamSynchronize(ctx->stream));
+        const double oldnorm = std::sqrt(std::max(0.0, hv[0].real()));
+        for (int i = 0; i <= j; ++i) b.H[(size_t)i * b.cap + j] += hv[1 + i] + hv[1 + (b.cap + 1) + i];
+        const double beta = std::sqrt(std::max(0.0, hv[1 + 2 * (b.cap + 1)].real()));
+        if (!std::isfinite(oldnorm) || !std::isfinite(beta)) throw Fail{RCMPS_ERR_SOLVER};
+        b.H[(size_t)(j + 1) * b.cap + j] = beta;
+        b.k = j + 1;
+        if (beta <= 10 * 2.220446049250313e-16 * std::max(oldnorm, 2.220446049250313e-16) || j + 1 == D2) {
+            b.exhausted = true;
+            break;
+        }
+        hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(j + 1) * n * 2);
+        ck(hipGetLastError());
+    }
+}
+
+// Projected shifted solves (I - z H_k) y = beta e1 for every node, by Givens QR of the Hessenberg
+// matrix; the Arnoldi residual estimate |z h_{k+1,k} y_k| / beta (kinetic_krylov.jl:95-113).
+bool projected_solutions(const KrylovBasis& b, const std::vector<double>& z, std::vector<cd>& Y, double& est) {
+    const int k = b.k, cap = b.cap, nz = (int)z.size();
+    Y.assign((size_t)k * nz, cd(0, 0));
+    est = 0.0;
+    std::vector<cd> A((size_t)k * k), g(k);
+    for (int i = 0; i < nz; ++i) {
+        for (int r = 0; r < k; ++r)
+            for (int c = 0; c < k; ++c)
+                A[(size_t)r * k + c] = (r == c ? cd(1, 0) : cd(0, 0)) - z[i] * (c >= r - 1 ? b.H[(size_t)r * cap + c] : cd(0, 0));
+        std::fill(g.begin(), g.end(), cd(0, 0));
+        g[0] = b.beta;
+        for (int c = 0; c + 1 < k; ++c) {
+            const cd x = A[(size_t)c * k + c], y = A[(size_t)(c + 1) * k + c];
+            const double ax = std::abs(x), ay = std::abs(y);
+            if (ay == 0) continue;
+            double cs;
+            cd sn;
+            if (ax == 0) { cs = 0; sn = std::conj(y) / ay; }
+            else { const double h = std::hypot(ax, ay); cs = ax / h; sn = (x / ax) * std::conj(y) / h; }
+            for (int col = c; col < k; ++col) {
+                const cd p = A[(size_t)c * k + col], q = A[(size_t)(c + 1) * k + col];
+                A[(size_t)c * k + col] = cs * p + sn * q;
+                A[(size_t)(c + 1) * k + col] = -std::conj(sn) * p + cs * q;
+            }
+            const cd p = g[c], q = g[c + 1];
+            g[c] = cs * p + sn * q;
+            g[c + 1] = -std::conj(sn) * p + cs * q;
+        }
+        for (int r = k - 1; r >= 0; --r) {
+            cd acc = g[r];
+            for (int c = r + 1; c < k; ++c) acc -= A[(size_t)r * k + c] * Y[(size_t)c + (size_t)k * i];
+            const cd d = A[(size_t)r * k + r];
+            if (std::abs(d) == 0) return false;
+            Y[(size_t)r + (size_t)k * i] = acc / d;
+        }
+        for (int r = 0; r < k; ++r)
+            if (!std::isfinite(Y[(size_t)r + (size_t)k * i].real()) || !std::isfinite(Y[(size_t)r + (size_t)k * i].imag()))
+                return false;
+        est = std::max(est, std::abs(z[i] * b.H[(size_t)k * cap + (k - 1)] * Y[(size_t)(k - 1) + (size_t)k * i]) / b.beta);
+    }
+    return true;
+}
+
+// Shifted solutions X_i = (I - z_i L)^{-1} rhs (or L*) for all nodes, each accepted on its true
+// relative residual (kinetic_krylov.jl:114-155).  X: nz matrices at stride Dp^2.
+void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasis& b, const double* rhs, bool adj,
+                       const std::vector<double>& z, double* X) {
+    KineticWork& kw = ctx->kin;
+    const rcmps_kinetic_options& o = ctx->kopt;
+    const int n = dv.n(), nz = (int)z.size();
+    const int maxdim = std::min(o.maxdim, st->D * st->D), initdim = std::min(o.initialdim, maxdim);
+    if (!b.started) basis_start(ctx, dv, b, rhs, initdim);
+    if (b.beta == 0.0) {
+        ck(hipMemsetAsync(X, 0, (size_t)nz * n * 16, ctx->stream));
+        return;
+    }
+    ck(kw.zd.ensure((size_t)nz * 8));
+    ck(kw.res.ensure((size_t)nz * 8));
+    ck(hipMemcpyAsync(kw.zd.d(), z.data(), (size_t)nz * 8, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<cd> Y;
+    std::vector<double> res(nz);
+    while (true) {
+        if (b.k == 0) basis_extend(ctx, st, dv, b, adj, initdim);
+        const int k = b.k;
+        double est = 0;
+        const bool ok = projected_solutions(b, z, Y, est);
+        const bool terminal = b.exhausted || k >= maxdim;
+        if (ok && (est <= o.tolerance || terminal)) {
+            ck(kw.Yd.ensure(Y.size() * 16));
+            ck(hipMemcpyAsync(kw.Yd.p, Y.data(), Y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(rc_basis_combine, dim3(ceil_div(n, 256), ceil_div(nz, 8)), dim3(256), 0, ctx->stream,
+                               b.V.d(), (long long)n, k, kw.Yd.d(), nz, n, X);
+            ck(hipGetLastError());
+            lindblad_batched(dv, st, X, kw.LX.d(), kw.T.d(), nz, adj);
+            hipLaunchKernelGGL(rc_shifted_residual, dim3(nz), dim3(256), 0, ctx->stream, X, kw.LX.d(), rhs, kw.zd.d(), n,
+                               kw.res.d());
+            ck(hipGetLastError());
+            ck(hipMemcpyAsync(res.data(), kw.res.d(), (size_t)nz * 8, hipMemcpyDeviceToHost, ctx->stream));
+            ck(hipStreamSynchronize(ctx->stream));
+            double worst = 0;
+            for (double r2 : res) worst = std::max(worst, std::sqrt(r2) / b.beta);
+            const bool fallback = terminal && worst > o.tolerance && worst <= o.fallback_tolerance;
+            if (std::isfinite(worst) && (worst <= o.tolerance || fallback)) {
+                kw.max_res = std::max(kw.max_res, worst);
+                return;
+            }
+            if (b.exhausted) throw Fail{RCMPS_ERR_SOLVER};
+        }
+        if (terminal) throw Fail{RCMPS_ERR_SOLVER};
+        basis_extend(ctx, st, dv, b, adj, std::min(2 * k, maxdim));
+    }
+}
+
+// One kinetic spec on the infinite line: value, covector and environment (kinetic_krylov.jl:156-189).
+// Scratch: 12 Dp x Dp matrices at kw.tmp.
+void kinetic_components(rcmps_context* ctx, const rcmps_state* st, Dev& dv, int commutator,
+                        const std::vector<double>& theta, const std::vector<double>& w, bool gradient, double& value,
+                        double* cov, double* env) {
+    KineticWork& kw = ctx->kin;
+    const int nz = (int)theta.size();
+    const long long node = (long long)dv.n();
+    const size_t mcount = (size_t)dv.n() * 2;
+    std::vector<double> z(nz), wz(nz);
+    for (int i = 0; i < nz; ++i) { z[i] = std::cos(theta[i]); wz[i] = w[i] * z[i]; }
+    ck(kw.wd.ensure((size_t)nz * 8));
+    ck(kw.wzd.ensure((size_t)nz * 8));
+    ck(hipMemcpyAsync(kw.wd.d(), w.data(), (size_t)nz * 8, hipMemcpyHostToDevice, ctx->stream));
+    ck(hipMemcpyAsync(kw.wzd.d(), wz.data(), (size_t)nz * 8, hipMemcpyHostToDevice, ctx->stream));
+    double* t = kw.tmp.d();
+    double* rhs = t;
+    double* u = t + mcount;
+    double* S1 = t + 2 * mcount;
+    double* S2 = t + 3 * mcount;
+    double* S3 = t + 4 * mcount;
+    double* S4 = t + 5 * mcount;
+    double* Sa = t + 6 * mcount;
+    double* a = t + 7 * mcount;
+    double* m = t + 8 * mcount;
+    double* nn = t + 9 * mcount;
+    double* v = t + 10 * mcount;
+    double* x2 = t + 11 * mcount;
+    const double* op = commutator ? st->C.d() : st->R.d();
+    const double* opd = commutator ? st->Ch.d() : st->Rh.d();
+    // forward solves on rho op^+
+    dv.gemm(0, st->rho.d(), 0, opd, rhs);
+    kinetic_solutions(ctx, st, dv, kw.basis[2 * commutator], rhs, false, z, kw.X.d());
+    hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, kw.X.d(), node, kw.wd.d(), nz, dv.n(), 0.0, u);
+    ck(hipGetLastError());
+    dv.trace_prod(op, u, ctx->small.d());
+    value = dv.fetch2(ctx->small.d()).real();   // sum_i w_i Re tr(op x_i)
+    if (!std::isfinite(value)) throw Fail{RCMPS_ERR_NONFINITE};
+    if (!gradient) return;
+    // adjoint solves on op^+; a_i = A_i^+
+    kinetic_solutions(ctx, st, dv, kw.basis[2 * commutator + 1], opd, true, z, kw.A.d());
+    GemmArgs mb{};   // T_i = R^+ A_i^+
+    mb.A = st->Rh.d(); mb.opB = 1; mb.B = kw.A.d(); mb.strideB = node; mb.C = kw.T.d(); mb.strideC = node;
+    dv.batched(mb, nz);
+    GemmArgs s1{};   // S1 = sum wz x_i T_i
+    s1.A = kw.X.d(); s1.strideA = node; s1.B = kw.T.d(); s1.strideB = node; s1.alpha = kw.wzd.d(); s1.alpha_stride = 1;
+    dv.reduce(s1, nz, S1, 1.0, 0.0);
+    GemmArgs s2{};   // S2 = sum wz x_i a_i
+    s2.A = kw.X.d(); s2.strideA = node; s2.opB = 1; s2.B = kw.A.d(); s2.strideB = node; s2.alpha = kw.wzd.d(); s2.alpha_stride = 1;
+    dv.reduce(s2, nz, S2, 1.0, 0.0);
+    GemmArgs nb{};   // T_i = R x_i
+    nb.A = st->R.d(); nb.B = kw.X.d(); nb.strideB = node; nb.C = kw.T.d(); nb.strideC = node;
+    dv.batched(nb, nz);
+    GemmArgs s3{};   // S3 = sum wz a_i T_i
+    s3.opA = 1; s3.A = kw.A.d(); s3.strideA = node; s3.B = kw.T.d(); s3.strideB = node; s3.alpha = kw.wzd.d(); s3.alpha_stride = 1;
+    dv.reduce(s3, nz, S3, 1.0, 0.0);
+    GemmArgs s4{};   // S4 = sum wz a_i x_i
+    s4.opA = 1; s4.A = kw.A.d(); s4.strideA = node; s4.B = kw.X.d(); s4.strideB = node; s4.alpha = kw.wzd.d(); s4.alpha_stride = 1;
+    dv.reduce(s4, nz, S4, 1.0, 0.0);
+    hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, kw.A.d(), node, kw.wd.d(), nz, dv.n(), 0.0, Sa);
+    ck(hipGetLastError());
+    dv.herm(1, 1.0, Sa, a);                                  // a = sum w a_i
+    dv.gemm(0, S2, 0, st->Rh.d(), x2);                       // m = S1 - S2 R^+
+    dv.axpby(cd(1, 0), S1, cd(-1, 0), x2, m);
+    dv.gemm(0, st->R.d(), 0, S4, x2);                        // nn = S3 - R S4
+    dv.axpby(cd(1, 0), S3, cd(-1, 0), x2, nn);
+    dv.gemm(0, a, 0, st->rho.d(), v);                        // v = a rho
+    dv.gemm(0, opd, 0, a, x2);                               // env = herm(op^+ a)
+    dv.herm(0, 1.0, x2, env);
+    if (commutator) {
+        dv.gemm(0, u, 0, st->Q.d(), m, 1.0, 1.0);            // m += u Q - Q u - R u R^+ + u R R^+
+        dv.gemm(0, st->Q.d(), 0, u, m, -1.0, 1.0);
+        dv.gemm(0, st->R.d(), 0, u, x2);
+        dv.gemm(0, x2, 0, st->Rh.d(), m, -1.0, 1.0);
+        dv.gemm(0, u, 0, st->R.d(), x2);
+        dv.gemm(0, x2, 0, st->Rh.d(), m, 1.0, 1.0);
+        dv.gemm(0, st->Qh.d(), 0, v, nn, 1.0, 1.0);          // nn += Q^+ v - v Q^+ - R v R^+ + R R^+ v
+        dv.gemm(0, v, 0, st->Qh.d(), nn, -1.0, 1.0);
+        dv.gemm(0, st->R.d(), 0, v, x2);
+        dv.gemm(0, x2, 0, st->Rh.d(), nn, -1.0, 1.0);
+        dv.gemm(0, st->Rh.d(), 0, v, x2);
+        dv.gemm(0, st->R.d(), 0, x2, nn, 1.0, 1.0);
+    } else {
+        dv.axpby(cd(1, 0), m, cd(1, 0), u, m);
+        dv.axpby(cd(1, 0), nn, cd(1, 0), v, nn);
+    }
+    dv.herm(1, 1.0, nn, x2);                                 // cov = (m + nn^+) / 2
+    dv.axpby(cd(0.5, 0), m, cd(0.5, 0), x2, cov);
+}
+
+// Finished Riemannian gradient of a covector/environment pair (observables.jl:201-212) into G (Dp x Dp)
+void finish_gradient_dev(rcmps_context* ctx, const rcmps_state* st, Dev& dv, const double* cov, const double* env,
+                         double metric_reg, double* G, double* scratch3, rcmps_stats* S) {
+    const size_t mcount = (size_t)dv.n() * 2;
+    double* h = scratch3;
+    double* c2 = scratch3 + mcount;
+    double* x = scratch3 + 2 * mcount;
+    boundary_solve(ctx, st, env, 1e-13, 1e-11, h, S);
+    dv.copy(cov, c2);
+    dv.gemm(0, st->rho.d(), 0, st->Rh.d(), x);
+    dv.gemm(0, x, 0, h, c2, 1.0, 1.0);
+    dv.gemm(0, st->rho.d(), 0, h, x);
+    dv.gemm(0, x, 0, st->Rh.d(), c2, -1.0, 1.0);
+    dv.gemm(1, c2, 0, metric_inverse(ctx, st, metric_reg), G);
+}
+
+// kinetic_resolvent (kinetic.jl:77-124) over the selected kinetic terms (bit 0: kinetic1, bit 1:
+// kinetic2) with weight one each: node doubling until energy and finished gradient settle.
+// cov, env (device, Dp-padded) receive the accepted level's raw covector and environment.
+void kinetic_resolvent_dev(rcmps_context* ctx, const rcmps_state* st, int which, double metric_reg, bool gradient,
+                           double& value, double* cov, double* env, rcmps_stats* S) {
+    const rcmps_kinetic_options& o = ctx->kopt;
+    KineticWork& kw = ctx->kin;
+    const int Dp = st->Dp, D = st->D;
+    Dev dv{ctx, Dp, D};
+    const size_t mb = (size_t)Dp * Dp * 16;
+    const size_t mcount = (size_t)Dp * Dp * 2;
+    for (auto& b : kw.basis) { b.k = 0; b.exhausted = false; b.started = false; b.beta = 0; std::fill(b.H.begin(), b.H.end(), cd(0, 0)); }
+    kw.actions = 0;
+    kw.max_res = 0;
+    ck(ctx->kry_w.ensure(mb));
+    ck(ctx->kry_t1.ensure(mb));
+    ck(ctx->small.ensure(64));
+    ck(kw.tmp.ensure(20 * mb));
+    const size_t batch_bytes = (size_t)o.maxnodes * mb;
+    ck(kw.X.ensure(batch_bytes));
+    ck(kw.A.ensure(batch_bytes));
+    ck(kw.LX.ensure(batch_bytes));
+    ck(kw.T.ensure(batch_bytes));
+    double* lev_c = kw.tmp.d() + 12 * mcount;    // current level covector / environment
+    double* lev_e = kw.tmp.d() + 13 * mcount;
+    double* c1 = kw.tmp.d() + 14 * mcount;       // one spec
+    double* e1 = kw.tmp.d() + 15 * mcount;
+    double* g = kw.tmp.d() + 16 * mcount;        // finished gradient of the previous level
+    double* gnew = kw.tmp.d() + 17 * mcount;
+    double* scr = kw.tmp.d() + 12 * mcount;      // overlaps nothing live during finish_gradient_dev? no: use extra
+    (void)scr;
+    DevBuf& fin = ctx->scratch;                  // 3 matrices for the finish
+    ck(fin.ensure(12 * mb));
+    auto level = [&](int nodes, double& val) {
+        std::vector<double> theta, w;
+        kinetic_grid(nodes, theta, w);
+        val = 0;
+        if (gradient) {
+            ck(hipMemsetAsync(lev_c, 0, mb, ctx->stream));
+            ck(hipMemsetAsync(lev_e, 0, mb, ctx->stream));
+        }
+        for (int comm = 0; comm < 2; ++comm) {
+            if (!(which & (1 << comm))) continue;
+            double v = 0;
+            kinetic_components(ctx, st, dv, comm, theta, w, gradient, v, c1, e1);
+            val += v;
+            if (gradient) {
+                dv.axpby(cd(1, 0), lev_c, cd(1, 0), c1, lev_c);
+                dv.axpby(cd(1, 0), lev_e, cd(1, 0), e1, lev_e);
+            }
+        }
+    };
+    int nodes = o.minnodes;
+    double prev = 0;
+    level(nodes, prev);
+    if (gradient) finish_gradient_dev(ctx, st, dv, lev_c, lev_e, metric_reg, g, fin.d(), S);
+    while (2 * nodes <= o.maxnodes) {
+        nodes *= 2;
+        double cur = 0;
+        level(nodes, cur);
+        const bool energy_pass = std::fabs(cur - prev) <= o.atol + o.rtol * std::fabs(cur);
+        bool done = false;
+        if (gradient) {
+            finish_gradient_dev(ctx, st, dv, lev_c, lev_e, metric_reg, gnew, fin.d(), S);
+            dv.axpby(cd(1, 0), gnew, cd(-1, 0), g, c1);
+            const double dg = dv.norm(c1), gn = dv.norm(gnew);
+            if (!std::isfinite(cur) || !std::isfinite(dg) || !std::isfinite(gn)) throw Fail{RCMPS_ERR_NONFINITE};
+            done = energy_pass && dg <= o.atol + o.rtol * gn;
+            dv.copy(gnew, g);
+        } else {
+            if (!std::isfinite(cur)) throw Fail{RCMPS_ERR_NONFINITE};
+            done = energy_pass;
+        }
+        prev = cur;
+        if (done) {
+            value = cur;
+            if (gradient) {
+                dv.copy(lev_c, cov);
+                dv.copy(lev_e, env);
+            }
+            if (S) {
+                S->kinetic_nodes = nodes;
+                for (int i = 0; i < 4; ++i) S->kinetic_basis[i] = kw.basis[i].k;
+                S->kinetic_actions = kw.actions;
+                S->kinetic_residual = kw.max_res;
+            }
+            return;
+        }
+    }
+    throw Fail{RCMPS_ERR_SOLVER};
 }
 
 bool valid_integrator(const rcmps_integrator* ip) {
@@ -1189,8 +1638,10 @@ int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcm
                        double metric_reg, int flags) {
     if (!ctx || !valid_integrator(ip) || !(metric_reg >= 0)) return RCMPS_ERR_ARGUMENT;
     if (std::sqrt(ip->max_point) >= RC_SOURCE_TMAX) return RCMPS_ERR_UNSUPPORTED;
+    const bool krylov = (flags & RCMPS_FLAG_KINETIC_KRYLOV) != 0;
     for (auto& r : reqs) {
         if (!r.st || !r.energy || !r.S || !std::isfinite(r.g)) return RCMPS_ERR_ARGUMENT;
+        if (krylov && (r.grad_R || r.grad_K)) return RCMPS_ERR_ARGUMENT;
         if (r.st->Dp / 16 > 4) return RCMPS_ERR_UNSUPPORTED;
         std::memset(r.S, 0, sizeof(*r.S));
     }
@@ -1203,11 +1654,28 @@ int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcm
             for (size_t i = 0; i < reqs.size(); ++i)
                 if (reqs[i].st->Dp == Dp) idx.push_back((int)i);
             if (idx.empty()) continue;
-            run_class(ctx, Dp, (flags & RCMPS_FLAG_SPLIT) != 0, ip, idx, reqs);
+            run_class(ctx, Dp, (flags & RCMPS_FLAG_SPLIT) != 0, krylov, ip, idx, reqs);
             for (size_t j = 0; j < idx.size(); ++j) {
                 EvalReq& r = reqs[idx[j]];
-                if (r.S->status != RCMPS_OK || !r.adjoint()) continue;
-                const int code = guard([&] { finish_state(ctx, r, *ctx->tb[j], ip, metric_reg, flags); });
+                if (r.S->status != RCMPS_OK) continue;
+                const int code = guard([&] {
+                    const double *kc = nullptr, *ke = nullptr;
+                    if (krylov) {
+                        // the kinetic job first, then the :phi^4: term (fixed order, observables.jl:236-246)
+                        const size_t mcount = (size_t)Dp * Dp * 2;
+                        ck(ctx->kin.tmp.ensure(20 * mcount * 8));
+                        double* kcov = ctx->kin.tmp.d() + 18 * mcount;
+                        double* kenv = ctx->kin.tmp.d() + 19 * mcount;
+                        double kv = 0;
+                        kinetic_resolvent_dev(ctx, r.st, 3, metric_reg, r.adjoint(), kv, kcov, kenv, r.S);
+                        *r.energy = kv + *r.energy;
+                        r.S->term_values[0] = kv;
+                        r.S->term_values[1] = 0.0;
+                        kc = kcov;
+                        ke = kenv;
+                    }
+                    if (r.adjoint()) finish_state(ctx, r, *ctx->tb[j], ip, metric_reg, flags, kc, ke);
+                });
                 if (code != RCMPS_OK) r.S->status = code;
             }
         }
@@ -1259,6 +1727,52 @@ int rcmps_fg_phi4_batch(rcmps_context* ctx, int n, const rcmps_state* const* sta
         reqs[i].S = stats ? stats + i : &local[i];
     }
     return fg_phi4_batch_impl(ctx, reqs, ip, metric_reg, flags);
+}
+
+int rcmps_set_kinetic_options(rcmps_context* ctx, const rcmps_kinetic_options* o) {
+    if (!ctx || !o) return RCMPS_ERR_ARGUMENT;
+    // src/kinetic.jl:78-82 and src/kinetic_krylov.jl:35-40
+    if (!(o->minnodes >= 2 && o->maxnodes >= 2 * o->minnodes && std::isfinite(o->atol) && std::isfinite(o->rtol) &&
+          o->atol > 0 && o->rtol >= 0 && std::isfinite(o->tolerance) && o->tolerance > 0 &&
+          std::isfinite(o->fallback_tolerance) && o->fallback_tolerance >= o->tolerance && o->initialdim >= 1 &&
+          o->maxdim >= o->initialdim))
+        return RCMPS_ERR_ARGUMENT;
+    ctx->kopt = *o;
+    return RCMPS_OK;
+}
+
+int rcmps_kinetic(rcmps_context* ctx, const rcmps_state* st, int which, double metric_reg, double* value,
+                  double* raw_cov, double* raw_env, double* grad, rcmps_stats* stats) {
+    if (!ctx || !st || !value || which < 1 || which > 3 || !(metric_reg >= 0)) return RCMPS_ERR_ARGUMENT;
+    rcmps_stats local{};
+    rcmps_stats* S = stats ? stats : &local;
+    std::memset(S, 0, sizeof(*S));
+    auto t0 = std::chrono::steady_clock::now();
+    const int rc = guard([&] {
+        ck(hipSetDevice(ctx->device));
+        const int D = st->D, Dp = st->Dp;
+        Dev dv{ctx, Dp, D};
+        const size_t mcount = (size_t)Dp * Dp * 2;
+        ck(ctx->kin.tmp.ensure(20 * mcount * 8));
+        double* kcov = ctx->kin.tmp.d() + 18 * mcount;
+        double* kenv = ctx->kin.tmp.d() + 19 * mcount;
+        const bool gradient = raw_cov || raw_env || grad;
+        double v = 0;
+        kinetic_resolvent_dev(ctx, st, which, metric_reg, gradient, v, kcov, kenv, S);
+        *value = v;
+        if (raw_cov) pad_out(ctx, D, Dp, kcov, raw_cov);
+        if (raw_env) pad_out(ctx, D, Dp, kenv, raw_env);
+        if (grad) {
+            ck(ctx->scratch.ensure(12 * mcount * 8));
+            double* G = ctx->kin.tmp.d() + 16 * mcount;
+            finish_gradient_dev(ctx, st, dv, kcov, kenv, metric_reg, G, ctx->scratch.d(), S);
+            pad_out(ctx, D, Dp, G, grad);
+        }
+        ck(hipStreamSynchronize(ctx->stream));
+    });
+    S->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    S->status = rc;
+    return rc;
 }
 
 int rcmps_fg_phi4_rk(rcmps_context* ctx, const rcmps_state* st, double g, const rcmps_integrator* ip, int flags,

@@ -35,3 +35,7 @@ __global__ void rc_add_identity(int Dp, int D, const double* cptr, double scale,
 __global__ void rc_sub_scaled(int n, const double* cptr, double dre, double dim, const double* Z, double* X);
 __global__ void rc_normalize(int n, const double* w, const double* nrm2, double* out);
 __global__ void rc_neg_add_identity(int Dp, int D, const double* cptr, double scale, double* Y);
+__global__ void rc_basis_combine(const double* V, long long stride, int k, const double* Y, int nnodes, int len,
+                                 double* X);
+__global__ void rc_shifted_residual(const double* X, const double* LX, const double* rhs, const double* z, int len,
+                                    double* out);

@@ -268,3 +268,62 @@ __global__ void rc_neg_add_identity(int Dp, int D, const double* cptr, double sc
     }
     y[i] = v;
 }
+
+// Krylov reconstruction X_i = sum_j Y[j + k i] V_j for nodes i (Y complex, one column per node);
+// each workgroup holds 256 elements of 8 nodes so that V is read once per 8 nodes.
+__global__ void __launch_bounds__(256) rc_basis_combine(const double* V, long long stride, int k, const double* Y,
+                                                        int nnodes, int len, double* X) {
+    __shared__ cplx ys[8][64];
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int i0 = blockIdx.y * 8;
+    const cplx* v = reinterpret_cast<const cplx*>(V);
+    const cplx* y = reinterpret_cast<const cplx*>(Y);
+    cplx acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = cmake(0, 0);
+    for (int j0 = 0; j0 < k; j0 += 64) {
+        for (int t = threadIdx.x; t < 8 * 64; t += 256) {
+            const int q = t / 64, jj = t % 64;
+            ys[q][jj] = (i0 + q < nnodes && j0 + jj < k) ? y[(size_t)(j0 + jj) + (size_t)k * (i0 + q)] : cmake(0, 0);
+        }
+        __syncthreads();
+        if (e < len) {
+            const int jn = k - j0 < 64 ? k - j0 : 64;
+            for (int jj = 0; jj < jn; ++jj) {
+                const cplx vv = v[(size_t)(j0 + jj) * stride + e];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[q] = cadd(acc[q], cmul(ys[q][jj], vv));
+            }
+        }
+        __syncthreads();
+    }
+    if (e < len) {
+        cplx* x = reinterpret_cast<cplx*>(X);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (i0 + q < nnodes) x[(size_t)(i0 + q) * len + e] = acc[q];
+    }
+}
+
+// out[i] = |X_i - z_i LX_i - rhs|^2 over len elements, one workgroup per node i
+__global__ void __launch_bounds__(256) rc_shifted_residual(const double* X, const double* LX, const double* rhs,
+                                                           const double* z, int len, double* out) {
+    __shared__ double s[256];
+    const int i = blockIdx.x, t = threadIdx.x;
+    const cplx* x = reinterpret_cast<const cplx*>(X) + (size_t)i * len;
+    const cplx* lx = reinterpret_cast<const cplx*>(LX) + (size_t)i * len;
+    const cplx* b = reinterpret_cast<const cplx*>(rhs);
+    const double zi = z[i];
+    double acc = 0;
+    for (int e = t; e < len; e += 256) {
+        const cplx r = csub(csub(x[e], cscale(zi, lx[e])), b[e]);
+        acc += cabs2(r);
+    }
+    s[t] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) s[t] += s[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[i] = s[0];
+}

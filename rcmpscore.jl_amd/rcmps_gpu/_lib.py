@@ -25,6 +25,7 @@ SYSTEMS = ("kinetic forward", "kinetic adjoint", "kinetic-commutator forward",
            "kinetic-commutator adjoint", "phi4 forward", "phi4 adjoint")
 FLAG_NO_BOUNDARY = 1
 FLAG_SPLIT = 2
+FLAG_KINETIC_KRYLOV = 4
 
 
 class SolverFailure(RuntimeError):
@@ -56,7 +57,9 @@ class Stats(ctypes.Structure):
                 ("trajectory_ms", ctypes.c_double), ("pullback_ms", ctypes.c_double),
                 ("boundary_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("krylov_matvecs", ctypes.c_int), ("krylov_residual", ctypes.c_double),
-                ("phase_us", (ctypes.c_double * 12) * 6)]
+                ("phase_us", (ctypes.c_double * 12) * 6),
+                ("kinetic_nodes", ctypes.c_int), ("kinetic_basis", ctypes.c_int * 4),
+                ("kinetic_actions", ctypes.c_int), ("kinetic_residual", ctypes.c_double)]
 
     def as_dict(self):
         out = {}
@@ -68,6 +71,14 @@ class Stats(ctypes.Structure):
                 out[name] = v
         out["status_name"] = STATUS.get(self.status, str(self.status))
         return out
+
+
+class KineticOptions(ctypes.Structure):
+    """rcmps_kinetic_options: KineticOptions (src/kinetic.jl:9-16) + KineticKrylovOptions
+    (src/kinetic_krylov.jl:8-13)."""
+    _fields_ = [("minnodes", ctypes.c_int), ("maxnodes", ctypes.c_int), ("atol", ctypes.c_double),
+                ("rtol", ctypes.c_double), ("tolerance", ctypes.c_double), ("fallback_tolerance", ctypes.c_double),
+                ("initialdim", ctypes.c_int), ("maxdim", ctypes.c_int)]
 
 
 # C ABI symbols: name -> (restype, argtypes)
@@ -87,6 +98,9 @@ SIGNATURES = {
     "rcmps_fg_phi4_batch": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(Integrator), ctypes.c_double, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_P), ctypes.POINTER(Stats)]),
+    "rcmps_set_kinetic_options": (ctypes.c_int, [_P, ctypes.POINTER(KineticOptions)]),
+    "rcmps_kinetic": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P, _P, _P,
+                                     ctypes.POINTER(Stats)]),
     "rcmps_fg_phi4_rk": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(Integrator), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double), _P, _P, ctypes.POINTER(Stats)]),
     "rcmps_boundary_environment": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Stats)]),

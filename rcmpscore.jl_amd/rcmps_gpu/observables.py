@@ -19,10 +19,18 @@ PARALLEL = (None, "serial", "terms", "trajectories", "both")
 PRECISIONS = ("fp64", "split")
 
 
-def _flags(boundary: bool, precision: str) -> int:
+KINETIC_BACKENDS = ("ode", "krylov")
+
+
+def _flags(boundary: bool, precision: str, kinetic_backend: str = "ode") -> int:
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {PRECISIONS}")
-    return (0 if boundary else _lib.FLAG_NO_BOUNDARY) | (_lib.FLAG_SPLIT if precision == "split" else 0)
+    if kinetic_backend not in KINETIC_BACKENDS:
+        if kinetic_backend == "resolvent":
+            raise NotImplementedError("the dense Schur kinetic backend is not built; use kinetic_backend='krylov'")
+        raise ValueError("unknown kinetic backend")
+    return ((0 if boundary else _lib.FLAG_NO_BOUNDARY) | (_lib.FLAG_SPLIT if precision == "split" else 0)
+            | (_lib.FLAG_KINETIC_KRYLOV if kinetic_backend == "krylov" else 0))
 
 
 def _integrator(p: IntegratorParameters, initial_step: float = 0.01) -> _lib.Integrator:
@@ -33,7 +41,8 @@ def _integrator(p: IntegratorParameters, initial_step: float = 0.01) -> _lib.Int
 
 
 def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: IntegratorParameters | None = None,
-                metric_reg: float | None = None, precision: str = "fp64"):
+                metric_reg: float | None = None, precision: str = "fp64", kinetic_backend: str = "ode",
+                kinetic_options=None, krylov_options=None):
     """(energy, gradient, raw covector, raw environment, stats dict) on device tensors.
 
     precision="split" runs the trajectory products on the INT8 MFMA over exact 8-bit slices of
@@ -51,7 +60,10 @@ def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: Integr
     env = torch.empty_like(G)
     e = ctypes.c_double()
     stats = _lib.Stats()
-    flags = _flags(boundary, precision)
+    flags = _flags(boundary, precision, kinetic_backend)
+    if kinetic_backend == "krylov":
+        from .kinetic import apply_options
+        apply_options(s.device, kinetic_options, krylov_options)
     st = lib.rcmps_fg_phi4(ctx, s.handle, float(g), ctypes.byref(ip), float(reg), flags, ctypes.byref(e),
                            G.data_ptr(), cov.data_ptr(), env.data_ptr(), ctypes.byref(stats))
     _lib.check(st, "fg_phi4", stats)
@@ -59,16 +71,16 @@ def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: Integr
 
 
 def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str = "ode", parallel=None,
-            boundary: bool = True, return_stats: bool = False, precision: str = "fp64"):
+            boundary: bool = True, return_stats: bool = False, precision: str = "fp64", kinetic_options=None,
+            krylov_options=None):
     """fg_phi4(s, g) -> (energy, gradient) with dE[W] = metric(s, gradient, W) for horizontal W.
 
     gradient=False returns the energy only.  `parallel` is accepted for signature compatibility:
     the six trajectories always run concurrently in one launch."""
     if parallel not in PARALLEL:
         raise ValueError("parallel must be :serial, :terms, :trajectories, or :both")
-    if kinetic_backend != "ode":
-        raise NotImplementedError("only the finite-window ODE kinetic backend is built")
-    energy, G, _, _, stats = fg_phi4_raw(s, g, boundary=boundary, precision=precision)
+    energy, G, _, _, stats = fg_phi4_raw(s, g, boundary=boundary, precision=precision, kinetic_backend=kinetic_backend,
+                                         kinetic_options=kinetic_options, krylov_options=krylov_options)
     if not gradient:
         return (energy, stats) if return_stats else energy
     grad = G.cpu().numpy()

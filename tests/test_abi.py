@@ -50,5 +50,21 @@ def test_package_import_does_not_touch_the_device():
 def test_flag_values_match_header():
     import rcmps_gpu._lib as L
     flags = dict(re.findall(r"#define RCMPS_FLAG_([A-Z_]+)\s+(\d+)", HEADER.read_text()))
-    assert flags == {"NO_BOUNDARY": "1", "SPLIT": "2"}
-    assert (L.FLAG_NO_BOUNDARY, L.FLAG_SPLIT) == (1, 2)
+    assert flags == {"NO_BOUNDARY": "1", "SPLIT": "2", "KINETIC_KRYLOV": "4"}
+    assert (L.FLAG_NO_BOUNDARY, L.FLAG_SPLIT, L.FLAG_KINETIC_KRYLOV) == (1, 2, 4)
+
+
+def test_struct_sizes_match_header():
+    """The ctypes mirrors of rcmps_integrator, rcmps_kinetic_options and rcmps_stats have the C sizes
+    (compiled against include/rcmps.h)."""
+    import subprocess, tempfile, os
+    import rcmps_gpu._lib as L
+    src = '#include <stdio.h>\n#include "rcmps.h"\nint main(void){printf("%zu %zu %zu\\n", sizeof(rcmps_integrator), ' \
+          'sizeof(rcmps_kinetic_options), sizeof(rcmps_stats)); return 0;}\n'
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "s.c")
+        exe = os.path.join(d, "s")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", str(ROOT / "include"), c, "-o", exe], check=True)
+        sizes = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert sizes == [ctypes.sizeof(L.Integrator), ctypes.sizeof(L.KineticOptions), ctypes.sizeof(L.Stats)]
