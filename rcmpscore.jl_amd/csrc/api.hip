@@ -677,10 +677,12 @@ int rcmps_state_create(rcmps_context* ctx, int D, const double* R, const double*
             double res = 0;
             auto A = [&](const double* v, double* w) { pre(v, tmp2); op(tmp2, w); };
             const int m = std::min(D * D, 40);
-            bool okg = gmres(dv, A, ctx->kry_b.d(), ctx->kry_x.d(), steady_tol, m, 200, mv, res);
+            // the reference iterates its CPTP Arnoldi to tol / 10 (src/transfer.jl:103-105)
+            bool okg = gmres(dv, A, ctx->kry_b.d(), ctx->kry_x.d(), steady_tol / 10, m, 200, mv, res);
             pre(ctx->kry_x.d(), st->rho.d());
             if (stats) { stats->krylov_matvecs = mv; stats->krylov_residual = res; }
-            if (!okg) throw Fail{RCMPS_ERR_SOLVER};
+            // accepted at steady_tol when the tighter target stagnates (as the boundary solve)
+            if (!okg && !(res <= steady_tol)) throw Fail{RCMPS_ERR_SOLVER};
             rho_h = download(ctx, D, Dp, st->rho.d());
         }
         if (!checked_density(D, qh, rh, rho_h, steady_tol)) throw Fail{RCMPS_ERR_SOLVER};

@@ -5,6 +5,7 @@ continuous MPS), its state preparation and optimizer, running on hand-written HI
 from ._lib import DeviceControlFailure, SolverFailure
 from .autograd import Phi4Energy, lbfgs_rk, phi4_energy
 from .kinetic import KineticKrylovOptions, KineticOptions, kinetic1, kinetic2
+from .jld2 import read_jld2
 from .io import StateRecord, rcmps_load, rcmps_store, read_portable, write_portable
 from .observables import (boundary_environment, fg_phi4, fg_phi4_batch, fg_phi4_raw, fg_phi4_rk,
                           trajectory_samples)
@@ -19,5 +20,5 @@ __all__ = [
     "ModelParameters", "RandomState", "RandomStateZ2", "SolverFailure", "State", "boundary_environment",
     "fg_phi4", "fg_phi4_raw", "metric", "optimize_state", "precondition", "random_state_arrays",
     "random_state_z2_arrays", "retraction", "trajectory_samples", "StateRecord", "rcmps_load", "rcmps_store",
-    "read_portable", "write_portable", "KineticKrylovOptions", "KineticOptions", "kinetic1", "kinetic2", "Phi4Energy", "fg_phi4_batch", "fg_phi4_rk", "lbfgs_rk", "phi4_energy",
+    "read_jld2", "read_portable", "write_portable", "KineticKrylovOptions", "KineticOptions", "kinetic1", "kinetic2", "Phi4Energy", "fg_phi4_batch", "fg_phi4_rk", "lbfgs_rk", "phi4_energy",
 ]

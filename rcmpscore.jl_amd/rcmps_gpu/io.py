@@ -269,11 +269,16 @@ def rcmps_store(s, path, *, overwrite: bool = True, **kw) -> Path:
     return write_portable(path, record_of_state(s, **kw), overwrite=overwrite)
 
 
-def rcmps_load(path, *, device: int = 0, recompute_rho: bool = False):
-    """Load a State from a portable directory; rho is checked if stored (recomputed otherwise)."""
+def rcmps_load(path, *, device: int = 0, recompute_rho: bool = False, steady_tol: float = 1e-11):
+    """Load a State from a portable directory or a JLD2 checkpoint (schema 1, src/storage.jl:27-38);
+    rho is checked if stored (recomputed otherwise; JLD2 checkpoints never store it)."""
     from .state import IntegratorParameters, ManifoldParameters, ModelParameters, State
 
-    rec = read_portable(path)
+    if str(path).endswith(".jld2"):
+        from .jld2 import read_jld2
+        rec = read_jld2(path)
+    else:
+        rec = read_portable(path)
     ip = IntegratorParameters()
     if rec.integration is not None:
         ip = IntegratorParameters(rec.integration.n_quad, rec.integration.max_point, rec.integration.atol,
@@ -283,7 +288,8 @@ def rcmps_load(path, *, device: int = 0, recompute_rho: bool = False):
         mp = ManifoldParameters(rec.manifold.metric_reg,
                                 rec.manifold.precond_reg_prefactor if rec.manifold.precond_reg_prefactor
                                 is not None else 1e-2, rec.manifold.precond_reg)
-    s = State(rec.R, rec.K, rho=None if recompute_rho else rec.rho, device=device, integrator_param=ip,
+    s = State(rec.R, rec.K, rho=None if recompute_rho else rec.rho, steady_tol=steady_tol, device=device,
+              integrator_param=ip,
               model_param=ModelParameters(rec.model, rec.coupling, rec.mass), manifold_param=mp,
               info=rec.info or "")
     if rec.energy is not None:

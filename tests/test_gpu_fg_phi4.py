@@ -135,3 +135,22 @@ def test_gradient_by_central_differences(rg):
         vals.append(rg.fg_phi4(t, 1.2, gradient=False))
     fd = (vals[0] - vals[1]) / (2 * h)
     assert rg.metric(s, g, w) == pytest.approx(fd, rel=2e-5, abs=1e-7)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_golden_d32_checkpoint(rg, precision):
+    """The reference's D=32, g=2.7 checkpoint (test/fixtures/phi4_g2.7_D32_z2_state.jld2): loaded
+    from JLD2 with its own integration settings, the energy matches its stored e0 and the gradient
+    its stored gradient.  The checkpoint holds no rho0, so both sides solve it; g = 2.7 is near the
+    critical coupling and rho0 (hence E and the tiny optimized gradient, |G| ~ 3e-6, through the
+    metric's (rho0 + 1e-3 I)^-1) moves by ~1e-10 in E and ~5e-7 in G as the stationary-state
+    tolerance goes from 1e-11 to 1e-13.  The bars are set at that floor; at D=16, where the
+    portable export stores rho0, the same comparison holds to 2e-11 (test_golden_fixture)."""
+    note(f"case: golden checkpoint D32 g=2.7 ({precision})")
+    s = rg.rcmps_load(FIXTURES / "phi4_g2.7_D32_z2_state.jld2")
+    assert s.integrator_param.n_quad == 192 and s.integrator_param.atol == 1e-13
+    e, g = rg.fg_phi4(s, s.model_param.coupling, precision=precision)
+    note(f"  E={e!r} stored e0={s.e0!r} diff={abs(e - s.e0):.2e} |dG|={np.abs(g - s.grad).max():.2e}")
+    assert abs(e - (-0.30190930265518967)) < 3e-10
+    assert np.abs(g - s.grad).max() < 1e-6
+    s.close()
