@@ -9,14 +9,15 @@ from .jld2 import read_jld2
 from .io import StateRecord, rcmps_load, rcmps_store, read_portable, write_portable
 from .observables import (boundary_environment, fg_phi4, fg_phi4_batch, fg_phi4_raw, fg_phi4_rk,
                           trajectory_samples)
-from .optimization import (BackwardLineSearch, metric, optimize_state, precondition, retraction)
+from .linesearch import BackwardLineSearch, HagerZhangLineSearch, SafeguardedLineSearch
+from .optimization import metric, optimize_state, precondition, retraction
 from .state import (IntegratorParameters, ManifoldParameters, ModelParameters, RandomState, RandomStateZ2,
                     State, random_state_arrays, random_state_z2_arrays)
 
 __version__ = "0.1.0"
 
 __all__ = [
-    "BackwardLineSearch", "DeviceControlFailure", "IntegratorParameters", "ManifoldParameters",
+    "BackwardLineSearch", "HagerZhangLineSearch", "SafeguardedLineSearch", "DeviceControlFailure", "IntegratorParameters", "ManifoldParameters",
     "ModelParameters", "RandomState", "RandomStateZ2", "SolverFailure", "State", "boundary_environment",
     "fg_phi4", "fg_phi4_raw", "metric", "optimize_state", "precondition", "random_state_arrays",
     "random_state_z2_arrays", "retraction", "trajectory_samples", "StateRecord", "rcmps_load", "rcmps_store",

@@ -3,17 +3,17 @@
 metric(s, W1, W2) = 2 Re tr(W1^+ W2 (rho0 + metric_reg I))                 (:1)
 precondition(s, W) = [(rho0 + precond_reg I)^{-1} (W (rho0 + metric_reg I))^+]^+   (:2-6)
 retraction(s, W, alpha) = State(R + alpha W, K + i alpha/2 (W^+ R - R^+ W))       (:7-13)
-optimize_state: L-BFGS (OptimKit LBFGS(m)) with the backtracking Armijo search of
-BackwardLineSearch (:25-59); failed trials (SolverFailure) are rejected (:38-51).
+optimize_state: L-BFGS (OptimKit LBFGS(m)) with the line search ls_default =
+SafeguardedLineSearch() (:54-106): Hager-Zhang with a bounded step and Armijo backtracking
+(BackwardLineSearch, :24-53) as recovery; failed trials (SolverFailure) are rejected.
 """
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import SolverFailure
+from .linesearch import SafeguardedLineSearch
 from .observables import fg_phi4
 from .state import State
 
@@ -49,34 +49,6 @@ def finalize(s: State, f, g):
     return s
 
 
-@dataclass
-class BackwardLineSearch:
-    """Bounded Armijo backtracking; a failed search never returns an uphill step (:25-59)."""
-    c1: float = 1e-4
-    shrink: float = 0.5
-    maxiter: int = 30
-
-    def __call__(self, fg, x, p, fg0, initialguess=1.0):
-        f0, g0 = fg0
-        slope = metric(x, g0, p)
-        if not slope < 0:
-            raise SolverFailure("line search direction is not a descent direction")
-        alpha = initialguess
-        failures, last = 0, ""
-        for k in range(1, self.maxiter + 1):
-            try:
-                y, v = retraction(x, p, alpha)
-                f, g = fg(y)
-                if math.isfinite(f) and f <= f0 + self.c1 * alpha * slope:
-                    return y, f, g, v, alpha, k
-            except SolverFailure as e:
-                failures += 1
-                last = str(e)
-            alpha *= self.shrink
-        detail = f"; {failures} numerical trial failures; last cause: {last}" if failures else ""
-        raise SolverFailure(f"Armijo line search exhausted {self.maxiter} steps{detail}")
-
-
 def energy_function(s: State):
     name, c = s.model_param.name, s.model_param.coupling
     if name == "phi4":
@@ -93,7 +65,7 @@ def optimize_state(s: State, *, method: str = "LBFGS", n_iter: int = 1000, gradt
     if method not in ("LBFGS", "GD"):
         raise ValueError(f"unknown optimization method: {method}")
     fg = energy_function(s)
-    ls = ls or BackwardLineSearch()
+    ls = ls or SafeguardedLineSearch()
     x = s
     f, g = fg(x)
     finalize(x, f, g)
@@ -121,7 +93,7 @@ def optimize_state(s: State, *, method: str = "LBFGS", n_iter: int = 1000, gradt
         if metric(x, g, p) >= 0:
             p = -precondition(x, g)
             S, Y, RHO = [], [], []
-        y, fy, gy, v, alpha, nfg = ls(fg, x, p, (f, g))
+        y, fy, gy, v, alpha, nfg = ls(fg, x, p, (f, g), retract=retraction, inner=metric)
         evaluations += nfg
         sk = alpha * v
         yk = gy - g
