@@ -19,6 +19,13 @@
  *   rcmps_boundary_environment  boundary_environment(s, a)                src/transfer.jl:122-153
  *   rcmps_trajectory_samples  the dense-output samples of one trajectory  src/observables.jl:77-96
  *                             (forward/adjoint trajectory solutions)
+ *   rcmps_excitation_create   ExcitationContext(s, p)                     src/excitation_environments.jl:18-40
+ *   rcmps_excitation_poisson  excitation_poisson, action_dual_poisson     src/excitation_environments.jl:75-85,
+ *                                                                          src/excitation_action.jl:85-94
+ *   rcmps_excitation_action   excitation_hamiltonian_action(ctx, W)       src/excitation_action.jl:194-228
+ *                             (excitation_kinetic_action_sum :135-179, excitation_observable_action :103-126);
+ *                             rcmps_gpu HamiltonianAction                  gpu/src/rcmps_gpu/excitations/
+ *                                                                          hamiltonian.py:21-72
  */
 #ifndef RCMPS_H
 #define RCMPS_H
@@ -51,6 +58,11 @@ extern "C" {
 
 typedef struct rcmps_context rcmps_context;
 typedef struct rcmps_state rcmps_state;
+typedef struct rcmps_excitation rcmps_excitation;
+
+/* parts of rcmps_excitation_action (0 = both) */
+#define RCMPS_EXC_KINETIC 1       /* the kinetic terms on the theta grid */
+#define RCMPS_EXC_POTENTIAL 2     /* the :phi^4: term by the action ODEs */
 
 /* IntegratorParameters (src/state.jl:4-10) plus the DOP853 initial step. */
 typedef struct {
@@ -156,6 +168,31 @@ int rcmps_boundary_environment(rcmps_context* ctx, const rcmps_state* st, const 
    (2 n_quad + 1 points x live channels x D x D, row-major) into a device buffer of
    `capacity` doubles.  Returns RCMPS_ERR_ARGUMENT if it is too small. */
 int rcmps_trajectory_samples(rcmps_context* ctx, int system, double* dst, long long capacity);
+
+/* ExcitationContext of a state at momentum p: the tangent-space excitation action works on it.  The
+   state must outlive the context.  No D^2 x D^2 transfer matrix is formed: the Poisson and resolvent
+   solves are matrix-free (GMRES and shifted Arnoldi), so the reference's dense mixing check is
+   replaced by the residual checks of those solves (a non-mixing state fails them with
+   RCMPS_ERR_SOLVER).  environment_tol is the reference's `tol` (1e-10). */
+int rcmps_excitation_create(rcmps_context* ctx, const rcmps_state* st, double momentum, double environment_tol,
+                            rcmps_excitation** out);
+void rcmps_excitation_destroy(rcmps_excitation* ex);
+
+/* adjoint = 0: x solves (-L + i shift p + |rho><I|) x = b - tr(b) rho, projected to tr(x) = 0
+   (excitation_poisson); adjoint = 1: (-L* - i shift p + |I><rho|) x = b - tr(rho b) I, projected to
+   tr(rho x) = 0 (action_dual_poisson).  shift in {-1, 0, 1}; b, x device D x D. */
+int rcmps_excitation_poisson(rcmps_context* ctx, rcmps_excitation* ex, const double* b, int adjoint, int shift,
+                             double* x);
+
+/* F_i = H(W_i) for n tangents W_i (device D x D; F_i device D x D) with <U, F(W)> the connected matrix
+   element <Phi_p(U)|H - E0|Phi_p(W)> per 2 pi delta(0).  The model is phi^4: kinetic_coef[0] (op R) and
+   kinetic_coef[1] (op [Q, R]) weight the kinetic terms (null: 1, 1) on the fixed theta grid of
+   kinetic_nodes Gauss nodes, and g weights :phi^4: integrated with `ip`.  `parts` selects
+   RCMPS_EXC_KINETIC and/or RCMPS_EXC_POTENTIAL (0: both; the kinetic preconditioner uses the kinetic
+   part alone).  All tangents of a call share their Krylov iterations and ODE steps. */
+int rcmps_excitation_action(rcmps_context* ctx, rcmps_excitation* ex, int n, const double* const* W, double* const* F,
+                            double g, const double* kinetic_coef, const rcmps_integrator* ip, int kinetic_nodes,
+                            int parts, rcmps_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -1799,3 +1799,5 @@ int rcmps_trajectory_samples(rcmps_context* ctx, int system, double* dst, long l
 }
 
 }  // extern "C"
+
+#include "excitation.inc"

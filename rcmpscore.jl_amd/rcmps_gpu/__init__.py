@@ -5,6 +5,10 @@ continuous MPS), its state preparation and optimizer, running on hand-written HI
 from ._lib import DeviceControlFailure, SolverFailure
 from .autograd import Phi4Energy, lbfgs_rk, phi4_energy
 from .kinetic import KineticKrylovOptions, KineticOptions, kinetic1, kinetic2
+from .excitation import (DavidsonOptions, ExcitationContext, ExcitationModel, ExcitationOperator,
+                         TangentCoordinates, action_dual_poisson, davidson, dispersion_relation, excitation_eigsolve,
+                         excitation_hamiltonian_action, excitation_parity, excitation_poisson, field_seed,
+                         iterative_dispersion, kinetic_preconditioner, relativistic_diagnostics)
 from .jld2 import read_jld2
 from .io import StateRecord, rcmps_load, rcmps_store, read_portable, write_portable
 from .observables import (boundary_environment, fg_phi4, fg_phi4_batch, fg_phi4_raw, fg_phi4_rk,
@@ -22,4 +26,8 @@ __all__ = [
     "fg_phi4", "fg_phi4_raw", "metric", "optimize_state", "precondition", "random_state_arrays",
     "random_state_z2_arrays", "retraction", "trajectory_samples", "StateRecord", "rcmps_load", "rcmps_store",
     "read_jld2", "read_portable", "write_portable", "KineticKrylovOptions", "KineticOptions", "kinetic1", "kinetic2", "Phi4Energy", "fg_phi4_batch", "fg_phi4_rk", "lbfgs_rk", "phi4_energy",
+    "DavidsonOptions", "ExcitationContext", "ExcitationModel", "ExcitationOperator", "TangentCoordinates",
+    "action_dual_poisson", "davidson", "dispersion_relation", "excitation_eigsolve", "excitation_hamiltonian_action",
+    "excitation_parity", "excitation_poisson", "field_seed", "iterative_dispersion", "kinetic_preconditioner",
+    "relativistic_diagnostics",
 ]

@@ -105,7 +105,16 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_double), _P, _P, ctypes.POINTER(Stats)]),
     "rcmps_boundary_environment": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Stats)]),
     "rcmps_trajectory_samples": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_longlong]),
+    "rcmps_excitation_create": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_P)]),
+    "rcmps_excitation_destroy": (None, [_P]),
+    "rcmps_excitation_poisson": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int, _P]),
+    "rcmps_excitation_action": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                               ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(Integrator), ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(Stats)]),
 }
+EXC_KINETIC = 1
+EXC_POTENTIAL = 2
 
 _lock = threading.Lock()
 _lib = None

@@ -1,0 +1,800 @@
+"""Tangent-space excitations on the GPU: the Hamiltonian action, white sector coordinates and the
+preconditioned Davidson eigensolver (src/excitation_environments.jl, src/excitations.jl,
+src/excitation_action.jl, src/davidson.jl, src/excitation_eigsolve.jl).
+
+The action F(W) (rcmps_excitation_action) runs on the device for whole batches of tangents; the
+white coordinates, the Davidson iteration and the kinetic preconditioner's dense eigenproblem act
+on sector vectors of length D^2/2 on the host, as in the reference.
+
+Implemented: ExcitationModel (:phi4), ExcitationContext, excitation_hamiltonian_action,
+excitation_poisson / action_dual_poisson, TangentCoordinates, DavidsonOptions, davidson,
+ExcitationOperator, kinetic_preconditioner, field_seed, excitation_eigsolve,
+iterative_dispersion, dispersion_relation (iterative), relativistic_diagnostics.
+Validation computes the two fresh-action energies of the reference's four (standard and refined
+settings); the two bilinear energies (the connected-element route) are not computed here, so
+`accepted` rests on the fresh energies and residuals alone.
+"""
+from __future__ import annotations
+
+import copy
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import SolverFailure
+from .kinetic import KineticOptions
+from .state import IntegratorParameters, State
+
+
+# --------------------------------------------------------------------------------------------
+# model and context
+# --------------------------------------------------------------------------------------------
+@dataclass(frozen=True)
+class ExcitationModel:
+    """The phi^4 Hamiltonian from the ground-state terms (src/excitations.jl:1-24): kinetic terms
+    with weights (1, 1) and g :phi^4:."""
+    name: str = "phi4"
+    coupling: float = 1.0
+
+    def __post_init__(self):
+        if self.name != "phi4":
+            raise NotImplementedError(f"excitation model {self.name!r}: only phi4 is built")
+        if not math.isfinite(self.coupling):
+            raise ValueError("nonfinite model parameter")
+
+    @property
+    def kinetic(self):
+        return (1.0, 1.0)
+
+    @staticmethod
+    def of(s: State) -> "ExcitationModel":
+        if s.model_param.name != "phi4":
+            raise NotImplementedError(f"excitation model {s.model_param.name!r}: only phi4 is built")
+        return ExcitationModel("phi4", float(np.real(s.model_param.coupling)))
+
+
+def check_kinetic_nodes(n):
+    if not (isinstance(n, (int, np.integer)) and n >= 2):
+        raise ValueError("kinetic_nodes must be an integer of at least 2")
+    return int(n)
+
+
+class ExcitationContext:
+    """ExcitationContext(s, p; tol) (src/excitation_environments.jl:18-40) on the device.  Built
+    matrix-free: the mixing check of the dense reference is replaced by the residual checks of the
+    Poisson and resolvent solves.  `integrator` overrides the state's integration settings for the
+    potential term (the refined settings of the validation)."""
+
+    def __init__(self, s: State, p: float = 0.0, *, tol: float = 1e-10, integrator=None):
+        if not (math.isfinite(p) and math.isfinite(tol) and tol > 0):
+            raise ValueError("invalid momentum or environment tolerance")
+        self.state = s
+        self.momentum = float(p)
+        self.tol = float(tol)
+        self.integrator = copy.deepcopy(integrator or s.integrator_param)
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.rcmps_excitation_create(_lib.context(s.device), s.handle, self.momentum, self.tol,
+                                               ctypes.byref(handle)), "excitation context")
+        self._handle = handle
+        _lib.register_state(self)
+
+    def with_settings(self, *, momentum=None, integrator=None) -> "ExcitationContext":
+        """ExcitationContext(base; momentum, integrator) (src/excitation_environments.jl:58-71)."""
+        return ExcitationContext(self.state, self.momentum if momentum is None else momentum, tol=self.tol,
+                                 integrator=integrator or self.integrator)
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def close(self):
+        if getattr(self, "_handle", None) is not None and not _lib.closed():
+            try:
+                _lib.load().rcmps_excitation_destroy(self._handle)
+            finally:
+                self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _device(s: State):
+    return torch.device("cuda", s.device)
+
+
+def _to_dev(x, s: State):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.complex128))).to(_device(s))
+
+
+def excitation_poisson(ctx: ExcitationContext, b, shift: int = 0):
+    """(-L + i shift p + |rho><I|) x = b - tr(b) rho, trace-projected (src/excitation_environments.jl:75-85)."""
+    return _poisson(ctx, b, False, shift)
+
+
+def action_dual_poisson(ctx: ExcitationContext, b, shifted: bool = False):
+    """(-L* - i p [shifted] + |I><rho|) x = b - tr(rho b) I, projected (src/excitation_action.jl:85-94)."""
+    return _poisson(ctx, b, True, 1 if shifted else 0)
+
+
+def _poisson(ctx, b, adjoint, shift):
+    s = ctx.state
+    bd = _to_dev(b, s)
+    xd = torch.empty_like(bd)
+    torch.cuda.synchronize(_device(s))
+    st = _lib.load().rcmps_excitation_poisson(_lib.context(s.device), ctx.handle, bd.data_ptr(), int(adjoint),
+                                              int(shift), xd.data_ptr())
+    _lib.check(st, "excitation Poisson solve")
+    return xd.cpu().numpy()
+
+
+def _integrator(p: IntegratorParameters):
+    return _lib.Integrator(int(p.n_quad), float(p.max_point), float(p.atol), float(p.rtol), int(p.maxiters), 0.01)
+
+
+def action_batch(ctx: ExcitationContext, ws, model: ExcitationModel, kinetic_nodes: int, parts: int = 0,
+                 return_stats: bool = False):
+    """F(W) for a list of D x D tangents in one device call (every kinetic Krylov iteration and ODE
+    step shared); kinetic part on the fixed grid of `kinetic_nodes` theta nodes."""
+    s = ctx.state
+    ws = [np.asarray(w, dtype=np.complex128) for w in ws]
+    for w in ws:
+        if w.shape != (s.D, s.D):
+            raise ValueError("tangent dimensions do not match the state")
+        if not np.isfinite(w).all():
+            raise ValueError("nonfinite tangent")
+    if not ws:
+        return ([], []) if return_stats else []
+    dev = _device(s)
+    W = torch.from_numpy(np.ascontiguousarray(np.stack(ws))).to(dev)
+    F = torch.empty_like(W)
+    n = len(ws)
+    wp = (ctypes.c_void_p * n)(*[W[i].data_ptr() for i in range(n)])
+    fp = (ctypes.c_void_p * n)(*[F[i].data_ptr() for i in range(n)])
+    kc = (ctypes.c_double * 2)(*model.kinetic)
+    ip = _integrator(ctx.integrator)
+    stats = _lib.Stats()
+    torch.cuda.synchronize(dev)
+    st = _lib.load().rcmps_excitation_action(_lib.context(s.device), ctx.handle, n, wp, fp, float(model.coupling), kc,
+                                             ctypes.byref(ip), int(kinetic_nodes), int(parts), ctypes.byref(stats))
+    _lib.check(st, "excitation action", stats)
+    out = list(F.cpu().numpy())
+    return (out, stats.as_dict()) if return_stats else out
+
+
+def excitation_hamiltonian_action(ctx: ExcitationContext, w, *, model: ExcitationModel | None = None,
+                                  kinetic_nodes=None, kinetic_options: KineticOptions | None = None,
+                                  parts: int = 0):
+    """excitation_hamiltonian_action(ctx, W) (src/excitation_action.jl:194-228): F with
+    <U, F(W)> = <Phi_p(U)|H - E0|Phi_p(W)>.  kinetic_nodes=None refines the theta grid by node
+    doubling for this W (src/excitation_action.jl:180-192); an integer fixes it.  `w` may be a list
+    of tangents (one batched device call; a list of actions is returned)."""
+    model = model or ExcitationModel.of(ctx.state)
+    single = not isinstance(w, (list, tuple))
+    ws = [w] if single else list(w)
+    if kinetic_nodes is not None:
+        out = action_batch(ctx, ws, model, check_kinetic_nodes(kinetic_nodes), parts)
+        return out[0] if single else out
+    o = kinetic_options or KineticOptions()
+    if not (o.minnodes >= 2 and o.maxnodes >= 2 * o.minnodes and o.atol > 0 and o.rtol >= 0):
+        raise ValueError("invalid kinetic quadrature controls")
+    pot = ([np.zeros((ctx.state.D, ctx.state.D), complex)] * len(ws) if parts == _lib.EXC_KINETIC else
+           action_batch(ctx, ws, model, 2, _lib.EXC_POTENTIAL))
+    if parts == _lib.EXC_POTENTIAL:
+        return pot[0] if single else pot
+    n = o.minnodes
+    prev = action_batch(ctx, ws, model, n, _lib.EXC_KINETIC)
+    while 2 * n <= o.maxnodes:
+        n *= 2
+        cur = action_batch(ctx, ws, model, n, _lib.EXC_KINETIC)
+        if all(np.linalg.norm(c - p) <= o.atol + o.rtol * np.linalg.norm(c) for c, p in zip(cur, prev)):
+            out = [c + q for c, q in zip(cur, pot)]
+            return out[0] if single else out
+        prev = cur
+    raise SolverFailure(f"excitation kinetic action quadrature did not converge by {n} nodes")
+
+
+# --------------------------------------------------------------------------------------------
+# white coordinates (src/excitation_eigsolve.jl:1-130)
+# --------------------------------------------------------------------------------------------
+def _parity_error(s: State, z):
+    zz = np.outer(z, z)
+    return max(np.linalg.norm(zz * s.R + s.R) / max(1, np.linalg.norm(s.R)),
+               np.linalg.norm(zz * s.K - s.K) / max(1, np.linalg.norm(s.K)), np.linalg.norm(zz * s.rho - s.rho))
+
+
+def excitation_parity(s: State, parity="auto", tol=1e-12):
+    """The diagonal Z2 parity of s as a +-1 vector, or None (src/excitation_eigsolve.jl:13-35)."""
+    if parity is None:
+        return None
+    if isinstance(parity, str):
+        if parity != "auto":
+            raise ValueError("parity must be 'auto', None, or a +-1 vector")
+        if s.D % 2:
+            return None
+        z = np.concatenate([np.ones(s.D // 2), -np.ones(s.D // 2)])
+        return z if _parity_error(s, z) <= tol else None
+    z = np.asarray(parity, dtype=float)
+    if not (len(z) == s.D and set(np.unique(z)) == {-1.0, 1.0}):
+        raise ValueError("parity must be a +-1 vector of length D with both signs")
+    if _parity_error(s, z) > tol:
+        raise ValueError("state is not Z2 symmetric under the supplied parity")
+    return z
+
+
+class TangentCoordinates:
+    """White coordinates of one sector: c -> W = X B^+ with X supported on `indices` and
+    B^+ rho B = 1 on the resolved metric support, so tr(W rho W^+) = |c|^2
+    (src/excitation_eigsolve.jl:37-112).  gauge='density' rotates into the eigenbasis of rho per
+    parity block; `state` is the state in the coordinates' gauge (a device State)."""
+
+    def __init__(self, s: State, *, sector="auto", parity="auto", gauge="density", metric_tol=0.0, parity_tol=1e-12,
+                 steady_tol=1e-11):
+        if not (math.isfinite(metric_tol) and 0 <= metric_tol < 1):
+            raise ValueError("metric_tol must lie in [0, 1)")
+        if gauge not in ("density", "original"):
+            raise ValueError("gauge must be 'density' or 'original'")
+        z = excitation_parity(s, parity, parity_tol)
+        if sector == "auto":
+            sector = "all" if z is None else "odd"
+        if sector not in ("all", "odd", "even"):
+            raise ValueError("sector must be 'auto', 'all', 'odd' or 'even'")
+        if sector != "all" and z is None:
+            raise ValueError(f"the {sector} sector requires a Z2-symmetric state; supply its parity")
+        d = s.D
+        blocks = [np.arange(d)] if z is None else [np.where(z == 1)[0], np.where(z == -1)[0]]
+        u = np.zeros((d, d), complex)
+        lam = np.zeros(d)
+        for b in blocks:
+            vals, vecs = np.linalg.eigh(s.rho[np.ix_(b, b)])
+            u[np.ix_(b, b)] = vecs
+            lam[b] = vals
+        cutoff = max(metric_tol, 10 * d * np.finfo(float).eps) * lam.max()
+        keep = lam > cutoff
+        if not keep.any():
+            raise SolverFailure("empty tangent metric support")
+        if gauge == "density":
+            rho = u.conj().T @ s.rho @ u
+            t = State(u.conj().T @ s.R @ u, u.conj().T @ s.K @ u, rho=(rho + rho.conj().T) / 2, steady_tol=steady_tol,
+                      device=s.device)
+            t.copy_param(s)
+            t.e0 = s.e0
+            rotation = u
+        else:
+            t, rotation = s, np.eye(d, dtype=complex)
+        B = np.zeros((d, d), complex)
+        colkeep = np.ones(d, bool)
+        for blk in blocks:
+            ix = np.ix_(blk, blk)
+            done = False
+            if keep.all():
+                try:
+                    L = np.linalg.cholesky(t.rho[ix])
+                    B[ix] = np.linalg.inv(L.conj().T)
+                    done = True
+                except np.linalg.LinAlgError:
+                    pass
+            if not done:
+                vals, vecs = np.linalg.eigh(t.rho[ix]) if gauge == "density" else (lam[blk], u[ix])
+                mask = vals > cutoff
+                B[ix] = vecs @ np.diag(mask / np.sqrt(np.maximum(vals, np.finfo(float).tiny)))
+                colkeep[blk] = mask
+        columns = np.where(colkeep)[0]
+        allowed = (lambda i, j: True) if sector == "all" else (lambda i, j: (sector == "odd") == (z[i] == z[j]))
+        self.indices = [(i, j) for j in columns for i in range(d) if allowed(i, j)]
+        if not self.indices:
+            raise SolverFailure("empty tangent sector")
+        self._rows = np.array([i for i, _ in self.indices])
+        self._cols = np.array([j for _, j in self.indices])
+        defect = np.linalg.norm(B.conj().T @ t.rho @ B - np.diag(np.isin(np.arange(d), columns).astype(float)))
+        if defect > 1e-6:
+            raise SolverFailure(f"tangent whitening is unresolved: metric defect {defect}")
+        self.state, self.rotation, self.whitening = t, rotation, B
+        self.parity, self.sector, self.gauge = z, sector, gauge
+        self.parity_error = math.nan if z is None else _parity_error(s, z)
+        self.metric_rank = int(keep.sum())
+        self.discarded_metric_weight = float(np.abs(lam[~keep]).sum())
+        self.metric_defect = float(defect)
+        self.density_condition = float(lam.max() / lam[keep].min())
+
+    def __len__(self):
+        return len(self.indices)
+
+    def tangent_matrix(self, x):
+        m = np.zeros((self.state.D, self.state.D), complex)
+        m[self._rows, self._cols] = x
+        return m @ self.whitening.conj().T
+
+    def tangent_coordinates(self, w):
+        return (np.asarray(w) @ self.state.rho @ self.whitening)[self._rows, self._cols]
+
+    def covector_coordinates(self, f):
+        return (np.asarray(f) @ self.whitening)[self._rows, self._cols]
+
+    def original_tangent(self, w):
+        return self.rotation @ w @ self.rotation.conj().T
+
+    def coordinate_tangent(self, w):
+        return self.rotation.conj().T @ w @ self.rotation
+
+
+# --------------------------------------------------------------------------------------------
+# Davidson (src/davidson.jl)
+# --------------------------------------------------------------------------------------------
+@dataclass
+class DavidsonOptions:
+    tol: float = 1e-7
+    maxops: int = 200
+    maxspace: int = 8
+    retain: int | None = None
+    refresh: bool = True
+    hermitian_tol: float = 1e-7
+    stall_window: int = 20
+    stall_minops: int = 40
+    stall_rtol: float = 0.1
+    stall_residual_limit: float | None = None
+    stall_energy_atol: float = 1e-9
+
+    def stall_limit(self):
+        return self.stall_residual_limit if self.stall_residual_limit is not None else max(1e-5, 10 * self.tol)
+
+
+def check_davidson(o: DavidsonOptions, nev):
+    retain = o.retain if o.retain is not None else nev
+    if nev < 1:
+        raise ValueError("nev must be positive")
+    if not (math.isfinite(o.tol) and o.tol > 0 and o.maxops >= 1 and math.isfinite(o.hermitian_tol)
+            and o.hermitian_tol > 0):
+        raise ValueError("invalid Davidson controls")
+    if not (nev <= retain < o.maxspace):
+        raise ValueError(f"require nev <= retain < maxspace (nev={nev}, retain={retain}, maxspace={o.maxspace})")
+    if o.maxops < nev:
+        raise ValueError("maxops must be at least nev")
+    if o.stall_window < 0:
+        raise ValueError("stall_window must be nonnegative")
+    if o.stall_window > 0:
+        limit = o.stall_limit()
+        if not (o.stall_minops >= 1 and 0 < o.stall_rtol < 1 and o.stall_energy_atol >= 0):
+            raise ValueError("invalid Davidson stagnation controls")
+        if not (math.isfinite(limit) and limit >= o.tol):
+            raise ValueError("stall_residual_limit must be at least tol")
+    return retain
+
+
+def _stalled(history, o: DavidsonOptions):
+    w = o.stall_window
+    if w <= 0 or not history:
+        return False
+    nev = len(history[-1]["energies"])
+    full = [r for r in history if len(r["energies"]) == nev]
+    n = len(full)
+    if n < max(o.stall_minops, w + 1):
+        return False
+    if full[-1]["residual"] > o.stall_limit():
+        return False
+    old_best = min(r["residual"] for r in full[:n - w])
+    best = min(r["residual"] for r in full)
+    if best < (1 - o.stall_rtol) * old_best:
+        return False
+    recent = full[n - w - 1:]
+    for k in range(nev):
+        vals = [r["energies"][k] for r in recent]
+        if max(vals) - min(vals) > o.stall_energy_atol:
+            return False
+    return True
+
+
+def _orthogonalize(x, basis):
+    for _ in range(2):
+        for b in basis:
+            x -= b * np.vdot(b, x)
+    return x
+
+
+def davidson(op, x0, *, nev=1, precondition=None, options: DavidsonOptions | None = None, callback=None,
+             batch_op=None):
+    """Lowest nev eigenpairs of a Hermitian map known through op(x) (src/davidson.jl:106-206):
+    restarted Rayleigh-Ritz with fresh or thick restarts, exact residuals from stored actions, the
+    stall rule and breakdown detection.  batch_op(list of x) -> list of actions, when given, applies
+    the operator to the start vectors together."""
+    o = options or DavidsonOptions()
+    retain = check_davidson(o, nev)
+    precondition = precondition or (lambda r: r)
+    starts = [x0] if np.ndim(x0) == 1 else list(x0)
+    if not starts:
+        raise ValueError("at least one start vector is required")
+    pending = []
+    for x in starts:
+        y = np.array(x, dtype=complex)
+        if not np.isfinite(y).all():
+            raise ValueError("start vectors must be finite")
+        _orthogonalize(y, pending)
+        scale = np.linalg.norm(x)
+        if scale > 0 and np.linalg.norm(y) > math.sqrt(np.finfo(float).eps) * scale:
+            pending.append(y / np.linalg.norm(y))
+    if not pending:
+        raise ValueError("start vectors must be nonzero and independent")
+    if len(pending) >= o.maxspace:
+        raise ValueError(f"at most maxspace-1={o.maxspace - 1} start vectors are accepted")
+    n = len(pending[0])
+    if nev > n:
+        raise ValueError(f"nev={nev} exceeds the dimension {n}")
+    stale = [None] * len(pending)
+    vs, avs, history = [], [], []
+    cycle = 1
+    prefetched = {}
+    count = 0
+    while count < o.maxops:
+        count += 1
+        x = pending.pop(0)
+        previous = stale.pop(0)
+        _orthogonalize(x, vs)
+        x /= np.linalg.norm(x)
+        if batch_op is not None and pending and not prefetched and len(vs) == 0:
+            # the start block: orthonormalise all pending vectors now and apply them together
+            block = [x]
+            for y in pending:
+                y = _orthogonalize(y.copy(), vs + block)
+                block.append(y / np.linalg.norm(y))
+            pending = block[1:]
+            acts = batch_op(block)
+            for y, a in zip(block[1:], acts[1:]):
+                prefetched[id(y)] = a
+            ax = np.asarray(acts[0], dtype=complex)
+        elif id(x) in prefetched:
+            ax = np.asarray(prefetched.pop(id(x)), dtype=complex)
+        else:
+            ax = np.asarray(op(x), dtype=complex)
+        if len(ax) != n or not np.isfinite(ax).all():
+            raise SolverFailure("invalid operator action")
+        change = 0.0 if previous is None else float(np.linalg.norm(ax - previous))
+        vs.append(x)
+        avs.append(ax)
+        V, AV = np.stack(vs, 1), np.stack(avs, 1)
+        P = V.conj().T @ AV
+        defect = np.linalg.norm(P - P.conj().T) / max(1.0, np.linalg.norm(P))
+        if defect > o.hermitian_tol:
+            raise SolverFailure(f"Davidson projected operator is not Hermitian: {defect}")
+        evals, evecs = np.linalg.eigh((P + P.conj().T) / 2)
+        m = min(nev, len(vs))
+        U, AU = V @ evecs[:, :m], AV @ evecs[:, :m]
+        for k in range(m):
+            sc = np.linalg.norm(U[:, k])
+            U[:, k] /= sc
+            AU[:, k] /= sc
+        energies = [float(np.vdot(U[:, k], AU[:, k]).real) for k in range(m)]
+        Rr = AU - U * np.array(energies)
+        residuals = [float(np.linalg.norm(Rr[:, k])) for k in range(m)]
+        row = dict(count=count, cycle=cycle, dimension=len(vs), energies=energies, residuals=residuals,
+                   energy=energies[0], residual=max(residuals), hermiticity=float(defect),
+                   orthogonality=float(np.linalg.norm(V.conj().T @ V - np.eye(len(vs)))), refresh_change=change)
+        history.append(row)
+        if callback is not None:
+            callback(row, U, AU)
+        converged = m == nev and row["residual"] <= o.tol
+        stalled = not converged and m == nev and _stalled(history, o)
+
+        def finish(reason):
+            return dict(energies=energies, vectors=[U[:, k].copy() for k in range(m)],
+                        actions=[AU[:, k].copy() for k in range(m)], residuals=residuals,
+                        converged=reason == "converged", stop_reason=reason, numops=count, history=history)
+        if converged:
+            return finish("converged")
+        if stalled:
+            return finish("stalled")
+        if count == o.maxops:
+            return finish("maxops")
+        if pending:
+            continue
+        if len(vs) >= o.maxspace:
+            keep = evecs[:, :min(retain, len(vs))]
+            kept, keptact = V @ keep, AV @ keep
+            cycle += 1
+            if o.refresh and o.maxops - count < nev:
+                return finish("maxops")
+            if o.refresh:
+                vs, avs = [], []
+                pending = [kept[:, k].copy() for k in range(kept.shape[1])]
+                stale = [keptact[:, k].copy() for k in range(kept.shape[1])]
+                continue
+            vs = [kept[:, k].copy() for k in range(kept.shape[1])]
+            avs = [keptact[:, k].copy() for k in range(kept.shape[1])]
+
+        def expansion(k):
+            y = -np.asarray(precondition(Rr[:, k]), dtype=complex)
+            if len(y) != n or not np.isfinite(y).all():
+                raise SolverFailure("invalid preconditioned residual")
+            _orthogonalize(y, vs)
+            if np.linalg.norm(y) < 1e-13 * max(1.0, residuals[k]):
+                y = Rr[:, k].copy()
+                _orthogonalize(y, vs)
+            return y if np.linalg.norm(y) > np.finfo(float).eps * max(1.0, residuals[k]) else None
+        candidates = [k for k in range(m) if residuals[k] > o.tol] or [m - 1]
+        y = None
+        for k in candidates:
+            y = expansion(k)
+            if y is not None:
+                break
+        if y is None:
+            if m == nev:
+                return finish("breakdown")
+            rng = np.random.default_rng(count)
+            y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            _orthogonalize(y, vs)
+        pending.append(y / np.linalg.norm(y))
+        stale.append(None)
+    raise AssertionError("unreachable")
+
+
+# --------------------------------------------------------------------------------------------
+# operator, preconditioner, start vectors, solve (src/excitation_eigsolve.jl:132-430)
+# --------------------------------------------------------------------------------------------
+class ExcitationOperator:
+    """x -> [F(W(x)) B]_sector in white coordinates; fixed kinetic_nodes make it exactly linear.
+    `calls` counts applications."""
+
+    def __init__(self, ctx: ExcitationContext, coordinates: TangentCoordinates, model: ExcitationModel, *,
+                 kinetic_nodes=256, kinetic_options=None):
+        if ctx.state.D != coordinates.state.D:
+            raise ValueError("context and coordinates differ")
+        self.context, self.coordinates, self.model = ctx, coordinates, model
+        self.kinetic_nodes = None if kinetic_nodes is None else check_kinetic_nodes(kinetic_nodes)
+        self.kinetic_options = kinetic_options or KineticOptions()
+        self.calls = 0
+
+    def apply_many(self, xs):
+        c = self.coordinates
+        ws = [c.tangent_matrix(x) for x in xs]
+        fs = excitation_hamiltonian_action(self.context, ws, model=self.model, kinetic_nodes=self.kinetic_nodes,
+                                           kinetic_options=self.kinetic_options)
+        self.calls += len(xs)
+        return [c.covector_coordinates(f) for f in fs]
+
+    def __call__(self, x):
+        return self.apply_many([x])[0]
+
+    def with_context(self, ctx, kinetic_nodes="same"):
+        return ExcitationOperator(ctx, self.coordinates, self.model,
+                                  kinetic_nodes=self.kinetic_nodes if kinetic_nodes == "same" else kinetic_nodes,
+                                  kinetic_options=self.kinetic_options)
+
+
+def kinetic_preconditioner(op: ExcitationOperator, *, nodes=8, floor=None, chunk=64):
+    """r -> sum_k v_k <v_k, r> / max(floor, kappa_k) from the coarse kinetic matrix on `nodes`
+    theta nodes (src/excitation_eigsolve.jl:181-201); one batched kinetic action per chunk."""
+    c, ctx = op.coordinates, op.context
+    n = len(c)
+    if op.model.kinetic == (0.0, 0.0):
+        raise ValueError("the kinetic preconditioner requires kinetic terms")
+    nodes = check_kinetic_nodes(nodes)
+    cols = []
+    eye = np.eye(n, dtype=complex)
+    for j0 in range(0, n, chunk):
+        ws = [c.tangent_matrix(eye[j]) for j in range(j0, min(n, j0 + chunk))]
+        fs = action_batch(ctx, ws, op.model, nodes, _lib.EXC_KINETIC)
+        cols += [c.covector_coordinates(f) for f in fs]
+    k = np.stack(cols, 1)
+    if not np.isfinite(k).all():
+        raise SolverFailure("nonfinite kinetic preconditioner")
+    vals, vecs = np.linalg.eigh((k + k.conj().T) / 2)
+    lower = max(op.model.kinetic) if floor is None else float(floor)
+    if not (math.isfinite(lower) and lower > 0):
+        raise ValueError("preconditioner floor must be positive")
+
+    def apply(r):
+        return vecs @ ((vecs.conj().T @ r) / np.maximum(lower, vals))
+    return dict(apply=apply, matrix=k, values=vals, vectors=vecs,
+                skew=float(np.linalg.norm(k - k.conj().T) / max(1.0, np.linalg.norm(k))), floor=lower, nodes=nodes)
+
+
+def field_seed(op: ExcitationOperator):
+    """White coordinates of the field insertion phi(0)|Psi>: W = 1 + [R, X] with
+    (L^+ + ip) X = R + R^+ trace-projected (src/excitation_eigsolve.jl:203-210)."""
+    s = op.context.state
+    x = -action_dual_poisson(op.context, s.R + s.R.conj().T, shifted=True)
+    return op.coordinates.tangent_coordinates(np.eye(s.D) + s.R @ x - x @ s.R)
+
+
+def initial_vectors(op, start, nev, pre):
+    c = op.coordinates
+    n = len(c)
+
+    def kinetic(k):
+        return [] if pre is None else [pre["vectors"][:, j].copy() for j in range(min(k, n))]
+    if isinstance(start, str):
+        if start not in ("auto", "field", "kinetic", "random"):
+            raise ValueError("start must be 'auto', 'field', 'kinetic', 'random', coordinates or tangents")
+        if start in ("auto", "field"):
+            x = field_seed(op)
+            if np.linalg.norm(x) > 1e-6:
+                return [x] + kinetic(nev - 1), "field"
+            if start == "field":
+                raise ValueError(f"the field insertion vanishes in the {c.sector} sector")
+        if start in ("auto", "kinetic") and pre is not None:
+            return kinetic(nev), "kinetic"
+        if start == "kinetic":
+            raise ValueError("start='kinetic' requires preconditioner='kinetic'")
+        rng = np.random.default_rng(1234)
+        return [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(nev)], "random"
+    starts = [start] if (isinstance(start, np.ndarray) and start.ndim in (1, 2)) else list(start)
+    out = []
+    for x in starts:
+        x = np.asarray(x, dtype=complex)
+        if x.ndim == 2:
+            if x.shape != (c.state.D, c.state.D):
+                raise ValueError("start tangent has the wrong size")
+            out.append(c.tangent_coordinates(c.coordinate_tangent(x)))
+        else:
+            if len(x) != n:
+                raise ValueError("start coordinates have the wrong length")
+            out.append(x)
+    return out, "supplied"
+
+
+def refined_integration(p: IntegratorParameters) -> IntegratorParameters:
+    """Window + 10, twice n_quad, tolerances / 10 (src/excitation_eigsolve.jl:250-252)."""
+    return IntegratorParameters(n_quad=2 * p.n_quad, max_point=p.max_point + 10, atol=p.atol / 10, rtol=p.rtol / 10,
+                                maxiters=p.maxiters)
+
+
+def fresh_level(op, x):
+    h = op(x)
+    value = np.vdot(x, h) / np.vdot(x, x)
+    return dict(energy=float(value.real), imaginary=float(value.imag),
+                residual=float(np.linalg.norm(h - value.real * x) / np.linalg.norm(x)), action=h)
+
+
+def solve_levels(coordinates, ctx, model, *, nev=1, kinetic_nodes=256, kinetic_options=None, preconditioner="kinetic",
+                 preconditioner_nodes=8, start="auto", davidson_options=None, validate=True, refined=None,
+                 refined_kinetic_nodes=None, validation_tol=1e-7, acceptance_residual=1e-5, callback=None,
+                 context_seconds=0.0):
+    """The iterative lowest-level solve (src/excitation_eigsolve.jl:302-401) without refinement
+    steps; validation uses the fresh standard and refined actions."""
+    n = len(coordinates)
+    if not (isinstance(nev, (int, np.integer)) and 1 <= nev <= n):
+        raise ValueError(f"nev={nev} must lie between 1 and the sector dimension {n}")
+    options = davidson_options or DavidsonOptions(maxspace=max(8, 2 * nev + 2))
+    check_davidson(options, nev)
+    if not (preconditioner in ("kinetic", "none") or callable(preconditioner)):
+        raise ValueError("preconditioner must be 'kinetic', 'none', or a function")
+    if not (acceptance_residual > 0 and validation_tol > 0):
+        raise ValueError("acceptance_residual and validation_tol must be positive")
+    op = ExcitationOperator(ctx, coordinates, model, kinetic_nodes=kinetic_nodes, kinetic_options=kinetic_options)
+    tick = time.time()
+    pre = kinetic_preconditioner(op, nodes=preconditioner_nodes) if preconditioner == "kinetic" else None
+    apply = pre["apply"] if pre is not None else (None if preconditioner == "none" else preconditioner)
+    pre_seconds = time.time() - tick
+    starts, startkind = initial_vectors(op, start, nev, pre)
+    tick = time.time()
+    res = davidson(op, starts, nev=nev, precondition=apply, options=options, callback=callback,
+                   batch_op=op.apply_many)
+    solve_seconds = time.time() - tick
+    coefficients = [v.copy() for v in res["vectors"]]
+    tick = time.time()
+    validation = None
+    refined_ctx = None
+    if validate:
+        rp = refined or refined_integration(ctx.integrator)
+        rn = refined_kinetic_nodes if refined_kinetic_nodes is not None else (
+            None if op.kinetic_nodes is None else 2 * op.kinetic_nodes)
+        refined_ctx = ctx.with_settings(integrator=rp)
+        reference = op.with_context(refined_ctx, kinetic_nodes=rn)
+        validation = []
+        for x in coefficients:
+            std, fine = fresh_level(op, x), fresh_level(reference, x)
+            spread = abs(std["energy"] - fine["energy"])
+            imag = max(abs(std["imaginary"]), abs(fine["imaginary"]))
+            residual = max(std["residual"], fine["residual"])
+            validation.append(dict(standard_energy=std["energy"], standard_residual=std["residual"],
+                                   refined_energy=fine["energy"], refined_residual=fine["residual"],
+                                   bilinear_energy=None, refined_bilinear_energy=None, energy_spread=spread,
+                                   max_imaginary=imag, residual_target_met=residual <= validation_tol,
+                                   consistent=spread <= validation_tol and imag <= validation_tol,
+                                   accepted=spread <= validation_tol and imag <= validation_tol
+                                   and residual <= acceptance_residual))
+    validation_seconds = time.time() - tick
+    accepted = len(coefficients) == nev and (res["converged"] if validation is None
+                                             else all(v["accepted"] for v in validation))
+    vectors = [coordinates.original_tangent(coordinates.tangent_matrix(x)) for x in coefficients]
+    return dict(energies=list(res["energies"]), vectors=vectors, coefficients=coefficients,
+                residuals=list(res["residuals"]), converged=res["converged"], stop_reason=res["stop_reason"],
+                accepted=accepted, nev=nev, validation_tol=validation_tol, acceptance_residual=acceptance_residual,
+                numops=res["numops"], action_calls=op.calls, history=res["history"], momentum=ctx.momentum,
+                sector=coordinates.sector, parity=coordinates.parity, gauge=coordinates.gauge, dimension=n,
+                metric_rank=coordinates.metric_rank, start=startkind, davidson=options,
+                preconditioner=None if pre is None else dict(nodes=pre["nodes"], skew=pre["skew"], floor=pre["floor"],
+                                                              minimum=float(pre["values"].min())),
+                validation=validation,
+                validated_energies=None if validation is None else [v["refined_energy"] for v in validation],
+                integration=ctx.integrator, refined_integration=None if refined_ctx is None else refined_ctx.integrator,
+                kinetic_nodes=op.kinetic_nodes,
+                timings=dict(context=context_seconds, preconditioner=pre_seconds, solve=solve_seconds,
+                             validation=validation_seconds),
+                model=model.name, coordinates=coordinates)
+
+
+def excitation_eigsolve(s: State, p: float = 0.0, *, model: ExcitationModel | None = None, sector="auto",
+                        parity="auto", parity_tol=1e-12, gauge="density", metric_tol=0.0, environment_tol=1e-10,
+                        max_bond=64, **kw):
+    """Lowest nev tangent-space excitation energies at momentum p by preconditioned Davidson on
+    device Hamiltonian actions (src/excitation_eigsolve.jl:430-447)."""
+    if not math.isfinite(p):
+        raise ValueError("invalid momentum")
+    if s.D > max_bond:
+        raise ValueError(f"excitation_eigsolve is limited to D<={max_bond}")
+    tick = time.time()
+    model = model or ExcitationModel.of(s)
+    c = TangentCoordinates(s, sector=sector, parity=parity, parity_tol=parity_tol, gauge=gauge, metric_tol=metric_tol)
+    ctx = ExcitationContext(c.state, p, tol=environment_tol)
+    return solve_levels(c, ctx, model, context_seconds=time.time() - tick, **kw)
+
+
+def iterative_dispersion(s: State, momenta, *, model=None, sector="auto", parity="auto", parity_tol=1e-12,
+                         gauge="density", metric_tol=0.0, environment_tol=1e-10, max_bond=64, start="auto",
+                         strict=True, **kw):
+    """excitation_eigsolve at every momentum on shared coordinates, each momentum warm-started from
+    the previous eigenvectors (src/excitation_eigsolve.jl:453-476)."""
+    if s.D > max_bond:
+        raise ValueError(f"excitation_eigsolve is limited to D<={max_bond}")
+    model = model or ExcitationModel.of(s)
+    c = TangentCoordinates(s, sector=sector, parity=parity, parity_tol=parity_tol, gauge=gauge, metric_tol=metric_tol)
+    results = []
+    for k, p in enumerate(momenta):
+        ctx = ExcitationContext(c.state, float(p), tol=environment_tol)
+        seed = start if (k == 0 or not (isinstance(start, str) and start == "auto")) else results[-1]["coefficients"]
+        results.append(solve_levels(c, ctx, model, start=seed, **kw))
+        if strict:
+            accepted_energies(results[-1])
+    return results
+
+
+def reported_energies(r):
+    return r["energies"] if r["validated_energies"] is None else r["validated_energies"]
+
+
+def accepted_energies(r):
+    if r["accepted"]:
+        return reported_energies(r)
+    raise SolverFailure(f"iterative excitation levels at p={r['momentum']} not accepted (stop: {r['stop_reason']})")
+
+
+def dispersion_relation(s: State, momenta, *, nev=1, strict=True, **kw):
+    """Energy-sorted levels at each momentum by the iterative solver (src/excitations.jl:227-246)."""
+    ps = [float(p) for p in momenta]
+    if not ps or not all(math.isfinite(p) for p in ps):
+        raise ValueError("momenta must be nonempty and finite")
+    results = iterative_dispersion(s, ps, nev=nev, strict=strict, **kw)
+    levels = [accepted_energies(r) if strict else reported_energies(r) for r in results]
+    if not all(len(l) == nev for l in levels):
+        raise SolverFailure(f"some momenta returned fewer than nev={nev} levels")
+    return dict(momenta=ps, energies=np.array(levels), spectra=results, accepted=[r["accepted"] for r in results])
+
+
+def relativistic_diagnostics(dispersion, *, branch=1, target_speed=1.0):
+    """E(p)^2 - c^2 p^2 and the least-squares fit E^2 = intercept + c^2 p^2 (src/excitations.jl:250-277)."""
+    if not (math.isfinite(target_speed) and target_speed > 0):
+        raise ValueError("target_speed must be finite and positive")
+    e = np.asarray(dispersion["energies"])
+    if not 1 <= branch <= e.shape[1]:
+        raise ValueError("invalid branch")
+    ps = np.asarray(dispersion["momenta"], dtype=float)
+    if len(np.unique(ps ** 2)) < 2:
+        raise ValueError("at least two distinct squared momenta are required")
+    energies = e[:, branch - 1]
+    if (energies < 0).any():
+        raise ValueError("negative excitation energies cannot be interpreted as a relativistic branch")
+    p2, y = ps ** 2, energies ** 2
+    a = np.stack([np.ones_like(ps), p2], 1)
+    (intercept, c2), *_ = np.linalg.lstsq(a, y, rcond=None)
+    invariant = y - target_speed ** 2 * p2
+    return dict(invariant_mass_squared=invariant, intercept=float(intercept), c_squared=float(c2),
+                fit_residual=float(np.linalg.norm(y - a @ np.array([intercept, c2])) / max(np.linalg.norm(y), 1e-300)),
+                invariant_spread=float(invariant.max() - invariant.min()))
