@@ -165,6 +165,11 @@ def action_batch(ctx: ExcitationContext, ws, model: ExcitationModel, kinetic_nod
     torch.cuda.synchronize(dev)
     st = _lib.load().rcmps_excitation_action(_lib.context(s.device), ctx.handle, n, wp, fp, float(model.coupling), kc,
                                              ctypes.byref(ip), int(kinetic_nodes), int(parts), ctypes.byref(stats))
+    if st != _lib.RCMPS_OK and st == 10:
+        phase = {20: "ket Poisson solve", 21: "kinetic forward Krylov", 22: "kinetic adjoint Krylov",
+                 23: "boundary Poisson solve", 24: "action ODE"}.get(stats.failed_system, "?")
+        raise SolverFailure(f"excitation action: solver failure in the {phase} (residual {stats.krylov_residual:.3e}, "
+                            f"Krylov dimension {stats.kinetic_basis[2]})")
     _lib.check(st, "excitation action", stats)
     out = list(F.cpu().numpy())
     return (out, stats.as_dict()) if return_stats else out

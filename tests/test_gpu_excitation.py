@@ -155,3 +155,26 @@ def test_d4_eigsolve_matches_golden_levels(rg, sector, p):
         assert abs(energy - e["ritz_energy"]) <= e["ritz_tolerance"], (level, energy, e["ritz_energy"])
         assert abs(v["standard_energy"] - e["four_energies"][0]) <= e["tolerance"]
         assert abs(v["refined_energy"] - e["four_energies"][1]) <= e["tolerance"]
+
+
+def test_d16_eigsolve_matches_golden_level(rg):
+    """The optimized D = 16 Z2 ground state (g = 2) at the settings of the reference's GPU comparison
+    runs: window 30, n_quad 512, tolerances 1e-13 / 1e-12, 256 kinetic nodes, thick restarts."""
+    note("case: excitation_eigsolve D=16 odd p=0 (gpu_comparison settings)")
+    from rcmps_gpu import DavidsonOptions, IntegratorParameters
+    from rcmps_gpu.jld2 import read_jld2
+    levels = json.loads((GOLDEN / "excitation_references.json").read_text())["levels"]
+    e = [x for x in levels if x["name"] == "D16_odd_p0"][0]
+    rec = read_jld2(GOLDEN / "phi4_g2_D16_z2_state.jld2")
+    s = rg.State(rec.R, rec.K, steady_tol=1e-12)
+    s.integrator_param = IntegratorParameters(n_quad=512, max_point=30.0, atol=1e-13, rtol=1e-12)
+    s.model_param.name = "phi4"
+    s.model_param.coupling = rec.coupling
+    opts = DavidsonOptions(tol=1e-7, maxops=200, maxspace=128, retain=12, refresh=False)
+    r = rg.excitation_eigsolve(s, 0.0, nev=1, kinetic_nodes=256, davidson_options=opts)
+    v = r["validation"][0]
+    note(f"  energy {r['energies'][0]!r} (golden {e['ritz_energy']!r}) numops {r['numops']} (golden {e['numops']}) "
+         f"refined {v['refined_energy']!r} (golden {e['four_energies'][1]!r}) timings {r['timings']}")
+    assert r["converged"] and r["sector"] == "odd" and r["dimension"] == e["dimension"]
+    assert abs(r["energies"][0] - e["ritz_energy"]) <= e["ritz_tolerance"]
+    assert abs(v["refined_energy"] - e["four_energies"][1]) <= e["ritz_tolerance"]
