@@ -8,6 +8,18 @@ struct RcStageCoef {
     int nk;
 };
 
+// arguments of rc_quartic_rhs (the state and derivative hold 4 blocks of 5 na matrices: forward x,
+// forward y, adjoint a0, adjoint a1; Vrep/Wrep hold v, w of tangent a at (5 a) Dp^2)
+struct RcQuarticArgs {
+    int Dp, na;
+    const double* U;
+    double* F;
+    const double *Q, *Qh, *Qs, *Qhs, *R, *Rh, *Vrep, *Wrep;
+    const double* coef;   // device [jac, jt] when non-null, else the values below
+    double jac, jt;
+};
+
+__global__ void rc_quartic_rhs(RcQuarticArgs g);
 __global__ void rc_sylvester_diag_c(int Dp, int D, int adjoint, double s_re, double s_im, const double* lam,
                                     const double* Y, double* Z);
 __global__ void rc_mdots(const double* V, long long sb, long long sv, int nk, const double* W, long long sw, int n,
@@ -16,8 +28,6 @@ __global__ void rc_msubtract(const double* V, long long sb, long long sv, int nk
                              double* W, long long sw);
 __global__ void rc_mnormalize(int n, const double* W, long long sw, const double* nrm2, int ldn, double* out,
                               long long so);
-__global__ void rc_chain(int n, int nch, int dir, double jt, double* F, const double* A, const double* B,
-                         const double* C);
 __global__ void rc_stage(long long N, const double* y, const double* K, long long sk, RcStageCoef c, double h,
                          double* out);
 __global__ void rc_step_error(int len, const double* y, const double* ynew, const double* K, long long sk,

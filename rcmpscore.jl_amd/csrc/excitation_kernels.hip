@@ -1,6 +1,6 @@
 // Device kernels of the tangent-space excitation action: the momentum-shifted no-jump
-// preconditioner, Gram-Schmidt over sets of Arnoldi bases, the channel chain of the :phi^4:
-// action ODE, DOP853 stage combinations and error norms, and batched trace products.
+// preconditioner, Gram-Schmidt over sets of Arnoldi bases, the fused right-hand side of the :phi^4:
+// action ODEs, DOP853 stage combinations and error norms, and batched trace products.
 #include "device_common.h"
 #include "excitation.h"
 
@@ -71,25 +71,6 @@ __global__ void rc_mnormalize(int n, const double* W, long long sw, const double
     const double s = q > 0.0 ? 1.0 / sqrt(q) : 0.0;
     reinterpret_cast<cplx*>(out)[(long long)b * so + i] =
         cscale(s, reinterpret_cast<const cplx*>(W)[(long long)b * sw + i]);
-}
-
-// Channel chain of the :phi^4: action: F[t, k] += c_k jt (A + B + C)[t, k - dir] with c_k = k for
-// the forward edges (dir = 1, k >= 1) and c_k = k + 1 for the reversed edges (dir = -1,
-// k <= nch - 2).  Matrices of trajectory t, channel k at (t nch + k) n.  grid (ceil(n/256), ntraj nch)
-__global__ void rc_chain(int n, int nch, int dir, double jt, double* F, const double* A, const double* B,
-                         const double* C) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tk = blockIdx.y, k = tk % nch;
-    if (i >= n) return;
-    const int src = k - dir;
-    if (src < 0 || src >= nch) return;
-    const double coef = (dir > 0 ? k : k + 1) * jt;
-    const long long o = (long long)(tk - dir) * n + i;
-    cplx s = reinterpret_cast<const cplx*>(A)[o];
-    if (B) s = cadd(s, reinterpret_cast<const cplx*>(B)[o]);
-    if (C) s = cadd(s, reinterpret_cast<const cplx*>(C)[o]);
-    cplx* f = reinterpret_cast<cplx*>(F) + (long long)tk * n + i;
-    *f = cadd(*f, cscale(coef, s));
 }
 
 // out = y + h sum_{j < c.nk} c.a[j] K_j  (N complex, K_j at K + j sk)
@@ -169,4 +150,156 @@ __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, doub
     if (i >= n) return;
     cplx* x = reinterpret_cast<cplx*>(X);
     x[i] = cadd(x[i], cmul(cmake(c_re, c_im), reinterpret_cast<const cplx*>(Z)[i]));
+}
+
+// ---------------------------------------------------------------------------------------------
+// One right-hand side of the :phi^4: action ODEs (forward and adjoint, every tangent and channel)
+// in a single launch.  Workgroup = one 16 x 16 output tile of one derivative matrix; all products
+// of that tile accumulate in one MFMA accumulator (k split across the 4 waves), the two-sided
+// products A X B through a Dp x 16 column block X B[:, tile] staged in LDS.
+//   forward  dx_k = jac (Q x + x Q^+ + R x R^+) + k jt (R x_{k-1} + x_{k-1} R^+)
+//            dy_k = jac (Qs y + y Q^+ + R y R^+ + v x + w x R^+) + k jt (R y_{k-1} + y_{k-1} R^+ + w x_{k-1})
+//   adjoint  da0_k = jac (Qhs a0 + a0 Q + R^+ (a0 R + a1 w) + a1 v) + (k+1) jt (a0 R + a1 w + R^+ a0)_{k+1}
+//            da1_k = jac (Q^+ a1 + a1 Q + R^+ a1 R) + (k+1) jt (a1 R + R^+ a1)_{k+1}
+// with Qs = Q - ip, Qhs = Q^+ + ip.  grid ((Dp/16)^2, 4 * 5 na).
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void qprod(cacc& acc, const cplx* A, const cplx* B, int Dp, int bi, int bj, int wave,
+                                      int lane, double scale) {
+    for (int ks = wave; ks < Dp / 4; ks += 4) {
+        const int kk = 4 * ks + (lane >> 4);
+        cplx a = A[(16 * bi + (lane & 15)) * Dp + kk];
+        a = cscale(scale, a);
+        const cplx b = B[kk * Dp + 16 * bj + (lane & 15)];
+        cmfma(acc, a, b);
+    }
+}
+
+// A[tile rows, :] T with T a Dp x 16 block in LDS
+__device__ __forceinline__ void qprod_t(cacc& acc, const cplx* A, const cplx* T, int Dp, int bi, int wave, int lane,
+                                        double scale) {
+    for (int ks = wave; ks < Dp / 4; ks += 4) {
+        const int kk = 4 * ks + (lane >> 4);
+        cplx a = A[(16 * bi + (lane & 15)) * Dp + kk];
+        a = cscale(scale, a);
+        cmfma(acc, a, T[kk * 16 + (lane & 15)]);
+    }
+}
+
+__device__ __forceinline__ cplx qreduce(const cacc& acc, cplx* red, int wave, int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave * 256 + (4 * q + (lane >> 4)) * 16 + (lane & 15)] = cacc_get(acc, q);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const cplx s = cadd(cadd(cadd(red[t], red[256 + t]), red[512 + t]), red[768 + t]);
+    __syncthreads();
+    return s;
+}
+
+// T = X1 B1[:, tile] (+ X2 B2[:, tile]) as a Dp x 16 block in LDS
+__device__ __forceinline__ void qcolumn(cplx* T, const cplx* X1, const cplx* B1, const cplx* X2, const cplx* B2,
+                                        int Dp, int bj, int wave, int lane, cplx* red) {
+    for (int rt = 0; rt < Dp / 16; ++rt) {
+        cacc acc;
+        cacc_zero(acc);
+        qprod(acc, X1, B1, Dp, rt, bj, wave, lane, 1.0);
+        if (X2) qprod(acc, X2, B2, Dp, rt, bj, wave, lane, 1.0);
+        const cplx v = qreduce(acc, red, wave, lane);
+        T[(16 * rt + (threadIdx.x >> 4)) * 16 + (threadIdx.x & 15)] = v;
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T1[64 * 16];
+    __shared__ cplx T2[64 * 16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int Dp = g.Dp, nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
+    const int nb = g.na * 5, mi = blockIdx.y, blk = mi / nb, r = mi % nb, a = r / 5, k = r % 5;
+    const long long m = (long long)Dp * Dp;
+    const cplx* U = reinterpret_cast<const cplx*>(g.U);
+    auto mat = [&](int b, int ch) { return U + ((long long)b * nb + a * 5 + ch) * m; };
+    const cplx* Q = reinterpret_cast<const cplx*>(g.Q);
+    const cplx* Qh = reinterpret_cast<const cplx*>(g.Qh);
+    const cplx* Qs = reinterpret_cast<const cplx*>(g.Qs);
+    const cplx* Qhs = reinterpret_cast<const cplx*>(g.Qhs);
+    const cplx* R = reinterpret_cast<const cplx*>(g.R);
+    const cplx* Rh = reinterpret_cast<const cplx*>(g.Rh);
+    const cplx* v = reinterpret_cast<const cplx*>(g.Vrep) + (long long)a * 5 * m;
+    const cplx* w = reinterpret_cast<const cplx*>(g.Wrep) + (long long)a * 5 * m;
+    const double jac = g.coef ? g.coef[0] : g.jac;
+    const double jt = g.coef ? g.coef[1] : g.jt;
+    cacc acc;
+    cacc_zero(acc);
+    if (blk == 0) {
+        const cplx* x = mat(0, k);
+        if (jac != 0.0) {
+            qprod(acc, Q, x, Dp, bi, bj, wave, lane, jac);
+            qprod(acc, x, Qh, Dp, bi, bj, wave, lane, jac);
+            qcolumn(T1, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
+        }
+        if (k >= 1) {
+            const cplx* xp = mat(0, k - 1);
+            qprod(acc, R, xp, Dp, bi, bj, wave, lane, k * jt);
+            qprod(acc, xp, Rh, Dp, bi, bj, wave, lane, k * jt);
+        }
+    } else if (blk == 1) {
+        const cplx* y = mat(1, k);
+        const cplx* x = mat(0, k);
+        if (jac != 0.0) {
+            qprod(acc, Qs, y, Dp, bi, bj, wave, lane, jac);
+            qprod(acc, y, Qh, Dp, bi, bj, wave, lane, jac);
+            qprod(acc, v, x, Dp, bi, bj, wave, lane, jac);
+            qcolumn(T1, y, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qcolumn(T2, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
+            qprod_t(acc, w, T2, Dp, bi, wave, lane, jac);
+        }
+        if (k >= 1) {
+            const cplx* yp = mat(1, k - 1);
+            const cplx* xp = mat(0, k - 1);
+            qprod(acc, R, yp, Dp, bi, bj, wave, lane, k * jt);
+            qprod(acc, yp, Rh, Dp, bi, bj, wave, lane, k * jt);
+            qprod(acc, w, xp, Dp, bi, bj, wave, lane, k * jt);
+        }
+    } else if (blk == 2) {
+        const cplx* a0 = mat(2, k);
+        const cplx* a1 = mat(3, k);
+        if (jac != 0.0) {
+            qprod(acc, Qhs, a0, Dp, bi, bj, wave, lane, jac);
+            qprod(acc, a0, Q, Dp, bi, bj, wave, lane, jac);
+            qprod(acc, a1, v, Dp, bi, bj, wave, lane, jac);
+            qcolumn(T1, a0, R, a1, w, Dp, bj, wave, lane, red);
+            qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac);
+        }
+        if (k <= 3) {
+            const double c = (k + 1) * jt;
+            const cplx* a0n = mat(2, k + 1);
+            const cplx* a1n = mat(3, k + 1);
+            qprod(acc, a0n, R, Dp, bi, bj, wave, lane, c);
+            qprod(acc, a1n, w, Dp, bi, bj, wave, lane, c);
+            qprod(acc, Rh, a0n, Dp, bi, bj, wave, lane, c);
+        }
+    } else {
+        const cplx* a1 = mat(3, k);
+        if (jac != 0.0) {
+            qprod(acc, Qh, a1, Dp, bi, bj, wave, lane, jac);
+            qprod(acc, a1, Q, Dp, bi, bj, wave, lane, jac);
+            qcolumn(T1, a1, R, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac);
+        }
+        if (k <= 3) {
+            const double c = (k + 1) * jt;
+            const cplx* a1n = mat(3, k + 1);
+            qprod(acc, a1n, R, Dp, bi, bj, wave, lane, c);
+            qprod(acc, Rh, a1n, Dp, bi, bj, wave, lane, c);
+        }
+    }
+    const cplx val = qreduce(acc, red, wave, lane);
+    reinterpret_cast<cplx*>(g.F)[(long long)mi * m + (16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15)] = val;
 }
