@@ -12,6 +12,7 @@ the same tolerance.
 """
 import json
 import math
+import os
 import sys
 from pathlib import Path
 
@@ -176,5 +177,30 @@ def test_d16_eigsolve_matches_golden_level(rg):
     note(f"  energy {r['energies'][0]!r} (golden {e['ritz_energy']!r}) numops {r['numops']} (golden {e['numops']}) "
          f"refined {v['refined_energy']!r} (golden {e['four_energies'][1]!r}) timings {r['timings']}")
     assert r["converged"] and r["sector"] == "odd" and r["dimension"] == e["dimension"]
+    assert abs(r["energies"][0] - e["ritz_energy"]) <= e["ritz_tolerance"]
+    assert abs(v["refined_energy"] - e["four_energies"][1]) <= e["ritz_tolerance"]
+
+
+@pytest.mark.skipif(os.environ.get("RCMPS_SLOW_EXCITATION") != "1",
+                    reason="several minutes on one MI355X: set RCMPS_SLOW_EXCITATION=1")
+@pytest.mark.parametrize("name", ["D32_odd_p0.05", "D32_even_p0"])
+def test_d32_eigsolve_matches_golden_level(rg, name):
+    """The optimized D = 32 Z2 ground state (g = 2.7) at the GPU-comparison settings."""
+    note(f"case: excitation_eigsolve {name} (gpu_comparison settings)")
+    from rcmps_gpu import DavidsonOptions, IntegratorParameters
+    from rcmps_gpu.jld2 import read_jld2
+    levels = json.loads((GOLDEN / "excitation_references.json").read_text())["levels"]
+    e = [x for x in levels if x["name"] == name][0]
+    rec = read_jld2(GOLDEN / "phi4_g2.7_D32_z2_state.jld2")
+    s = rg.State(rec.R, rec.K, steady_tol=1e-12)
+    s.integrator_param = IntegratorParameters(n_quad=512, max_point=30.0, atol=1e-13, rtol=1e-12)
+    s.model_param.name = "phi4"
+    s.model_param.coupling = rec.coupling
+    opts = DavidsonOptions(tol=1e-7, maxops=200, maxspace=160, retain=12, refresh=False)
+    r = rg.excitation_eigsolve(s, e["momentum"], nev=1, sector=e["sector"], kinetic_nodes=256, davidson_options=opts)
+    v = r["validation"][0]
+    note(f"  energy {r['energies'][0]!r} (golden {e['ritz_energy']!r}) numops {r['numops']} (golden {e['numops']}) "
+         f"refined {v['refined_energy']!r} (golden {e['four_energies'][1]!r}) timings {r['timings']}")
+    assert r["converged"] and r["dimension"] == e["dimension"]
     assert abs(r["energies"][0] - e["ritz_energy"]) <= e["ritz_tolerance"]
     assert abs(v["refined_energy"] - e["four_energies"][1]) <= e["ritz_tolerance"]
