@@ -8,6 +8,11 @@ whose inputs are resident in HBM.  `value` is the split-precision path (trajecto
 the INT8 MFMA over exact 8-bit slices); the plain FP64 path is timed the same way beside it, and
 the split result's energy and gradient errors against it are reported.  --batch N instead times
 the batched workload (N states, D uniform in {8, 16, 24, 32, 48, 64}, g uniform in [0.5, 4]).
+--excitation instead times the tangent-space excitation dispersion: a Z2-symmetric D=32 state
+(RandomStateZ2 distribution, scaling 0.2, seed 42), g = 2, the odd sector, nev = 2 at the nine
+momenta 0, 0.25, ..., 2 with their Davidson searches in lockstep (every device call applies one
+action per momentum), 256 kinetic nodes, default integration, validated by fresh actions at the
+refined settings.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -141,6 +146,55 @@ def batch_main(args):
     return 0
 
 
+def excitation_main(args):
+    """One step = the whole lockstep dispersion solve (preconditioners, Davidson, validation)."""
+    import numpy as np
+    import torch
+
+    import rcmps_gpu
+    D = args.exc_D
+    r, k = rcmps_gpu.random_state_z2_arrays(D, scaling=0.2, seed=42)
+    s = rcmps_gpu.State(r, k)
+    s.model_param.name = "phi4"
+    s.model_param.coupling = 2.0
+    momenta = [0.25 * i for i in range(9)]
+    rows = []
+    for step in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t_start = time.perf_counter()
+        rs = rcmps_gpu.iterative_dispersion(
+            s, momenta, nev=2, sector="odd", lockstep=True, strict=False, preconditioner_chunk=args.exc_chunk,
+            progress=lambda msg: print(f"[{time.perf_counter() - t_start:7.1f} s] {msg}", file=sys.stderr, flush=True))
+        torch.cuda.synchronize()
+        rows.append((time.perf_counter() - t0, rs))
+    timed = rows[args.warmup:]
+    elapsed = sum(t for t, _ in timed) / len(timed)
+    rs = timed[-1][1]
+    actions = sum(r["action_calls"] for r in rs)
+    out = {
+        "metric": "excitation dispersion wall time (9 momenta x nev=2, Davidson searches in lockstep)",
+        "value": elapsed, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"RandomStateZ2({D}) scaling 0.2 seed 42, phi4 g=2, odd sector, nev=2, p=0:0.25:2, "
+                               "kinetic_nodes=256, n_quad=128, max_point=20, atol=1e-11, rtol=1e-10, lockstep",
+                   "sector_dimension": rs[0]["dimension"]},
+        "actions": actions, "actions_per_s": actions / elapsed,
+        "device_rounds": rs[0]["device_rounds"],
+        "timings": rs[0]["timings"],
+        "levels": [{"p": r["momentum"], "energies": r["energies"], "numops": r["numops"],
+                    "residuals": r["residuals"], "accepted": r["accepted"],
+                    "refined": [v["refined_energy"] for v in r["validation"]],
+                    "spread": max(v["energy_spread"] for v in r["validation"])} for r in rs],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+    torch.cuda.synchronize()
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +209,10 @@ def main(argv=None):
                          "calibrates HBM byte counters in a profile of the same command)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="time the batched workload of N states instead of the single D=64 evaluation")
+    ap.add_argument("--excitation", action="store_true",
+                    help="time the lockstep excitation dispersion (9 momenta, nev=2) instead")
+    ap.add_argument("--exc-D", type=int, default=32)
+    ap.add_argument("--exc-chunk", type=int, default=128, help="tangents per coarse kinetic action call")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -162,6 +220,8 @@ def main(argv=None):
         return 2
     if args.batch > 0:
         return batch_main(args)
+    if args.excitation:
+        return excitation_main(args)
     import numpy as np
     import torch
 

@@ -194,6 +194,14 @@ int rcmps_excitation_action(rcmps_context* ctx, rcmps_excitation* ex, int n, con
                             double g, const double* kinetic_coef, const rcmps_integrator* ip, int kinetic_nodes,
                             int parts, rcmps_stats* stats);
 
+/* rcmps_excitation_action with one context per tangent: contexts of different momenta on one state, so
+   that the Davidson searches of several momenta (iterative_dispersion, src/excitation_eigsolve.jl:453-476)
+   apply their actions in one call.  All exs[i] must belong to the same state (RCMPS_ERR_ARGUMENT
+   otherwise).  No counterpart in the reference, which applies one W at one momentum per call. */
+int rcmps_excitation_action_multi(rcmps_context* ctx, int n, rcmps_excitation* const* exs, const double* const* W,
+                                  double* const* F, double g, const double* kinetic_coef, const rcmps_integrator* ip,
+                                  int kinetic_nodes, int parts, rcmps_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,13 +10,27 @@ struct RcStageCoef {
 
 // arguments of rc_quartic_rhs (the state and derivative hold 4 blocks of 5 na matrices: forward x,
 // forward y, adjoint a0, adjoint a1; Vrep/Wrep hold v, w of tangent a at (5 a) Dp^2)
+// The epilogue chains the DOP853 stages: epi = 1 writes the next stage input
+// Unext = y + h (sum_{j<s} comb_j K_j + comb_s F) for this tile's elements (or the step end with
+// the B weights), epi = 2 writes this tile's error sums sum |E5.K/sc|^2, sum |E3.K/sc|^2 with
+// sc = atol + rtol max(|y|, |U|) (U is then the step end) into partial[2 (mi ntiles + tile)].
 struct RcQuarticArgs {
     int Dp, na;
     const double* U;
     double* F;
     const double *Q, *Qh, *Qs, *Qhs, *R, *Rh, *Vrep, *Wrep;
+    long long qstride;    // complex stride between the tangents' Qs (and Qhs); 0 when shared
     const double* coef;   // device [jac, jt] when non-null, else the values below
     double jac, jt;
+    int epi, stage;
+    const double* y;      // state at the step start
+    const double* K;      // stage derivatives, K_j at K + j sk (complex stride)
+    long long sk;
+    RcStageCoef comb, comb3;
+    double* Unext;
+    const double* hp;     // device step size when non-null, else h
+    double h, atol, rtol;
+    double* partial;
 };
 
 __global__ void rc_quartic_rhs(RcQuarticArgs g);
@@ -29,9 +43,7 @@ __global__ void rc_msubtract(const double* V, long long sb, long long sv, int nk
 __global__ void rc_mnormalize(int n, const double* W, long long sw, const double* nrm2, int ldn, double* out,
                               long long so);
 __global__ void rc_stage(long long N, const double* y, const double* K, long long sk, RcStageCoef c, double h,
-                         double* out);
-__global__ void rc_step_error(int len, const double* y, const double* ynew, const double* K, long long sk,
-                              RcStageCoef e5, RcStageCoef e3, double atol, double rtol, double* out);
+                         const double* hp, double* out);
 __global__ void rc_trace_prod_batched(int Dp, const double* A, long long sa, const double* B, long long sb,
                                       double* out);
 __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, double* X);

@@ -73,51 +73,18 @@ __global__ void rc_mnormalize(int n, const double* W, long long sw, const double
         cscale(s, reinterpret_cast<const cplx*>(W)[(long long)b * sw + i]);
 }
 
-// out = y + h sum_{j < c.nk} c.a[j] K_j  (N complex, K_j at K + j sk)
+// out = y + h sum_{j < c.nk} c.a[j] K_j  (N complex, K_j at K + j sk); h from hp when non-null
 __global__ void rc_stage(long long N, const double* y, const double* K, long long sk, RcStageCoef c, double h,
-                         double* out) {
+                         const double* hp, double* out) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    if (hp) h = hp[0];
     const cplx* kk = reinterpret_cast<const cplx*>(K);
     cplx s = cmake(0, 0);
     for (int j = 0; j < c.nk; ++j)
         if (c.a[j] != 0.0) s = cadd(s, cscale(c.a[j], kk[(long long)j * sk + i]));
     const cplx yy = reinterpret_cast<const cplx*>(y)[i];
     reinterpret_cast<cplx*>(out)[i] = cadd(yy, cscale(h, s));
-}
-
-// DOP853 error sums of one segment of `len` complex elements per workgroup (segment = blockIdx.x):
-// out[2 s] = sum |E5.K / sc|^2, out[2 s + 1] = sum |E3.K / sc|^2 with sc = atol + rtol max(|y|, |ynew|)
-__global__ void __launch_bounds__(256) rc_step_error(int len, const double* y, const double* ynew, const double* K,
-                                                     long long sk, RcStageCoef e5, RcStageCoef e3, double atol,
-                                                     double rtol, double* out) {
-    __shared__ double s5[256], s3[256];
-    const int seg = blockIdx.x, t = threadIdx.x;
-    const long long base = (long long)seg * len;
-    const cplx* kk = reinterpret_cast<const cplx*>(K);
-    const cplx* y0 = reinterpret_cast<const cplx*>(y);
-    const cplx* y1 = reinterpret_cast<const cplx*>(ynew);
-    double a5 = 0, a3 = 0;
-    for (int e = t; e < len; e += 256) {
-        const long long i = base + e;
-        cplx r5 = cmake(0, 0), r3 = cmake(0, 0);
-        for (int j = 0; j < e5.nk; ++j) {
-            const cplx kj = kk[(long long)j * sk + i];
-            if (e5.a[j] != 0.0) r5 = cadd(r5, cscale(e5.a[j], kj));
-            if (e3.a[j] != 0.0) r3 = cadd(r3, cscale(e3.a[j], kj));
-        }
-        const double sc = atol + rtol * fmax(cabsd(y0[i]), cabsd(y1[i]));
-        a5 += cabs2(r5) / (sc * sc);
-        a3 += cabs2(r3) / (sc * sc);
-    }
-    s5[t] = a5;
-    s3[t] = a3;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (t < h) { s5[t] += s5[t + h]; s3[t] += s3[t + h]; }
-        __syncthreads();
-    }
-    if (t == 0) { out[2 * seg] = s5[0]; out[2 * seg + 1] = s3[0]; }
 }
 
 // out[b] = tr(A_b B_b) for Dp x Dp matrices; grid (nb)
@@ -161,7 +128,7 @@ __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, doub
 //            dy_k = jac (Qs y + y Q^+ + R y R^+ + v x + w x R^+) + k jt (R y_{k-1} + y_{k-1} R^+ + w x_{k-1})
 //   adjoint  da0_k = jac (Qhs a0 + a0 Q + R^+ (a0 R + a1 w) + a1 v) + (k+1) jt (a0 R + a1 w + R^+ a0)_{k+1}
 //            da1_k = jac (Q^+ a1 + a1 Q + R^+ a1 R) + (k+1) jt (a1 R + R^+ a1)_{k+1}
-// with Qs = Q - ip, Qhs = Q^+ + ip.  grid ((Dp/16)^2, 4 * 5 na).
+// with Qs = Q - ip, Qhs = Q^+ + ip of each tangent's momentum.  grid ((Dp/16)^2, 4 * 5 na).
 // ---------------------------------------------------------------------------------------------
 namespace {
 
@@ -225,8 +192,8 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
     auto mat = [&](int b, int ch) { return U + ((long long)b * nb + a * 5 + ch) * m; };
     const cplx* Q = reinterpret_cast<const cplx*>(g.Q);
     const cplx* Qh = reinterpret_cast<const cplx*>(g.Qh);
-    const cplx* Qs = reinterpret_cast<const cplx*>(g.Qs);
-    const cplx* Qhs = reinterpret_cast<const cplx*>(g.Qhs);
+    const cplx* Qs = reinterpret_cast<const cplx*>(g.Qs) + (long long)a * g.qstride;     // tangent's momentum
+    const cplx* Qhs = reinterpret_cast<const cplx*>(g.Qhs) + (long long)a * g.qstride;
     const cplx* R = reinterpret_cast<const cplx*>(g.R);
     const cplx* Rh = reinterpret_cast<const cplx*>(g.Rh);
     const cplx* v = reinterpret_cast<const cplx*>(g.Vrep) + (long long)a * 5 * m;
@@ -301,5 +268,38 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
         }
     }
     const cplx val = qreduce(acc, red, wave, lane);
-    reinterpret_cast<cplx*>(g.F)[(long long)mi * m + (16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15)] = val;
+    const long long e = (long long)mi * m + (16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15);
+    reinterpret_cast<cplx*>(g.F)[e] = val;
+    if (g.epi == 0) return;
+    const cplx* Kc = reinterpret_cast<const cplx*>(g.K);
+    const cplx y0 = reinterpret_cast<const cplx*>(g.y)[e];
+    if (g.epi == 1) {
+        const double h = g.hp ? g.hp[0] : g.h;
+        cplx sum = cscale(g.comb.a[g.stage], val);
+        for (int j = 0; j < g.stage; ++j)
+            if (g.comb.a[j] != 0.0) sum = cadd(sum, cscale(g.comb.a[j], Kc[(long long)j * g.sk + e]));
+        reinterpret_cast<cplx*>(g.Unext)[e] = cadd(y0, cscale(h, sum));
+        return;
+    }
+    cplx r5 = cscale(g.comb.a[g.stage], val), r3 = cscale(g.comb3.a[g.stage], val);
+    for (int j = 0; j < g.stage; ++j) {
+        const cplx kj = Kc[(long long)j * g.sk + e];
+        if (g.comb.a[j] != 0.0) r5 = cadd(r5, cscale(g.comb.a[j], kj));
+        if (g.comb3.a[j] != 0.0) r3 = cadd(r3, cscale(g.comb3.a[j], kj));
+    }
+    const double sc = g.atol + g.rtol * fmax(cabsd(y0), cabsd(U[e]));
+    double* s5 = reinterpret_cast<double*>(red);
+    double* s3 = s5 + 256;
+    s5[tid] = cabs2(r5) / (sc * sc);
+    s3[tid] = cabs2(r3) / (sc * sc);
+    __syncthreads();
+    for (int hh = 128; hh > 0; hh >>= 1) {
+        if (tid < hh) { s5[tid] += s5[tid + hh]; s3[tid] += s3[tid + hh]; }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long long slot = (long long)mi * nbt * nbt + blockIdx.x;
+        g.partial[2 * slot] = s5[0];
+        g.partial[2 * slot + 1] = s3[0];
+    }
 }

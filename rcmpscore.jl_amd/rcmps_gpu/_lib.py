@@ -112,6 +112,10 @@ SIGNATURES = {
                                                ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(Integrator), ctypes.c_int, ctypes.c_int,
                                                ctypes.POINTER(Stats)]),
+    "rcmps_excitation_action_multi": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                                     ctypes.POINTER(_P), ctypes.c_double,
+                                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(Integrator),
+                                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)]),
 }
 EXC_KINETIC = 1
 EXC_POTENTIAL = 2

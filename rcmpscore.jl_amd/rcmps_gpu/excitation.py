@@ -27,7 +27,7 @@ import torch
 
 from . import _lib
 from ._lib import SolverFailure
-from .kinetic import KineticOptions
+from .kinetic import KineticKrylovOptions, KineticOptions, apply_options
 from .state import IntegratorParameters, State
 
 
@@ -140,32 +140,52 @@ def _integrator(p: IntegratorParameters):
     return _lib.Integrator(int(p.n_quad), float(p.max_point), float(p.atol), float(p.rtol), int(p.maxiters), 0.01)
 
 
-def action_batch(ctx: ExcitationContext, ws, model: ExcitationModel, kinetic_nodes: int, parts: int = 0,
-                 return_stats: bool = False):
+# Krylov controls of the theta-resolvent solves: the action's (true relative residual 1e-12), and
+# the coarse kinetic matrix of the preconditioner's (1e-9: a preconditioner needs no more)
+ACTION_KRYLOV = KineticKrylovOptions()
+PRECONDITIONER_KRYLOV = KineticKrylovOptions(tolerance=1e-9, fallback_tolerance=1e-9)
+
+
+def action_batch(ctx, ws, model: ExcitationModel, kinetic_nodes: int, parts: int = 0, return_stats: bool = False,
+                 krylov_options: KineticKrylovOptions | None = None):
     """F(W) for a list of D x D tangents in one device call (every kinetic Krylov iteration and ODE
-    step shared); kinetic part on the fixed grid of `kinetic_nodes` theta nodes."""
-    s = ctx.state
+    step shared); kinetic part on the fixed grid of `kinetic_nodes` theta nodes.  `ctx` is one
+    ExcitationContext or one per tangent (momenta may differ; state and integration settings must
+    not)."""
     ws = [np.asarray(w, dtype=np.complex128) for w in ws]
+    ctxs = list(ctx) if isinstance(ctx, (list, tuple)) else [ctx] * len(ws)
+    if len(ctxs) != len(ws):
+        raise ValueError("one context per tangent is required")
+    if not ws:
+        return ([], {}) if return_stats else []
+    s = ctxs[0].state
+    ip0 = ctxs[0].integrator
+    for c in ctxs:
+        if c.state is not s:
+            raise ValueError("all contexts of one call must share the state")
+        if c.integrator != ip0:
+            raise ValueError("all contexts of one call must share the integration settings")
     for w in ws:
         if w.shape != (s.D, s.D):
             raise ValueError("tangent dimensions do not match the state")
         if not np.isfinite(w).all():
             raise ValueError("nonfinite tangent")
-    if not ws:
-        return ([], []) if return_stats else []
     dev = _device(s)
     W = torch.from_numpy(np.ascontiguousarray(np.stack(ws))).to(dev)
     F = torch.empty_like(W)
     n = len(ws)
     wp = (ctypes.c_void_p * n)(*[W[i].data_ptr() for i in range(n)])
     fp = (ctypes.c_void_p * n)(*[F[i].data_ptr() for i in range(n)])
+    hp = (ctypes.c_void_p * n)(*[c.handle for c in ctxs])
     kc = (ctypes.c_double * 2)(*model.kinetic)
-    ip = _integrator(ctx.integrator)
+    ip = _integrator(ip0)
     stats = _lib.Stats()
+    apply_options(s.device, None, krylov_options or ACTION_KRYLOV)
     torch.cuda.synchronize(dev)
-    st = _lib.load().rcmps_excitation_action(_lib.context(s.device), ctx.handle, n, wp, fp, float(model.coupling), kc,
-                                             ctypes.byref(ip), int(kinetic_nodes), int(parts), ctypes.byref(stats))
-    if st != _lib.RCMPS_OK and st == 10:
+    st = _lib.load().rcmps_excitation_action_multi(_lib.context(s.device), n, hp, wp, fp, float(model.coupling), kc,
+                                                   ctypes.byref(ip), int(kinetic_nodes), int(parts),
+                                                   ctypes.byref(stats))
+    if st == 10:
         phase = {20: "ket Poisson solve", 21: "kinetic forward Krylov", 22: "kinetic adjoint Krylov",
                  23: "boundary Poisson solve", 24: "action ODE"}.get(stats.failed_system, "?")
         raise SolverFailure(f"excitation action: solver failure in the {phase} (residual {stats.krylov_residual:.3e}, "
@@ -181,8 +201,10 @@ def excitation_hamiltonian_action(ctx: ExcitationContext, w, *, model: Excitatio
     """excitation_hamiltonian_action(ctx, W) (src/excitation_action.jl:194-228): F with
     <U, F(W)> = <Phi_p(U)|H - E0|Phi_p(W)>.  kinetic_nodes=None refines the theta grid by node
     doubling for this W (src/excitation_action.jl:180-192); an integer fixes it.  `w` may be a list
-    of tangents (one batched device call; a list of actions is returned)."""
-    model = model or ExcitationModel.of(ctx.state)
+    of tangents (one batched device call; a list of actions is returned), and `ctx` a list of
+    contexts, one per tangent (momenta may differ)."""
+    s0 = (ctx[0] if isinstance(ctx, (list, tuple)) else ctx).state
+    model = model or ExcitationModel.of(s0)
     single = not isinstance(w, (list, tuple))
     ws = [w] if single else list(w)
     if kinetic_nodes is not None:
@@ -191,7 +213,7 @@ def excitation_hamiltonian_action(ctx: ExcitationContext, w, *, model: Excitatio
     o = kinetic_options or KineticOptions()
     if not (o.minnodes >= 2 and o.maxnodes >= 2 * o.minnodes and o.atol > 0 and o.rtol >= 0):
         raise ValueError("invalid kinetic quadrature controls")
-    pot = ([np.zeros((ctx.state.D, ctx.state.D), complex)] * len(ws) if parts == _lib.EXC_KINETIC else
+    pot = ([np.zeros((s0.D, s0.D), complex)] * len(ws) if parts == _lib.EXC_KINETIC else
            action_batch(ctx, ws, model, 2, _lib.EXC_POTENTIAL))
     if parts == _lib.EXC_POTENTIAL:
         return pot[0] if single else pot
@@ -410,6 +432,22 @@ def davidson(op, x0, *, nev=1, precondition=None, options: DavidsonOptions | Non
     restarted Rayleigh-Ritz with fresh or thick restarts, exact residuals from stored actions, the
     stall rule and breakdown detection.  batch_op(list of x) -> list of actions, when given, applies
     the operator to the start vectors together."""
+    gen = davidson_steps(x0, nev=nev, precondition=precondition, options=options, callback=callback,
+                         blocks=batch_op is not None)
+    apply = batch_op or (lambda xs: [op(x) for x in xs])
+    try:
+        request = next(gen)
+        while True:
+            request = gen.send(apply(request))
+    except StopIteration as stop:
+        return stop.value
+
+
+def davidson_steps(x0, *, nev=1, precondition=None, options: DavidsonOptions | None = None, callback=None,
+                   blocks=False):
+    """The Davidson iteration of `davidson` as a generator: it yields lists of vectors to apply and
+    receives their actions (so that several searches can share device calls); its return value is
+    davidson's result.  With blocks, the start vectors of a (re)started space are applied together."""
     o = options or DavidsonOptions()
     retain = check_davidson(o, nev)
     precondition = precondition or (lambda r: r)
@@ -443,21 +481,21 @@ def davidson(op, x0, *, nev=1, precondition=None, options: DavidsonOptions | Non
         previous = stale.pop(0)
         _orthogonalize(x, vs)
         x /= np.linalg.norm(x)
-        if batch_op is not None and pending and not prefetched and len(vs) == 0:
+        if blocks and pending and not prefetched and len(vs) == 0:
             # the start block: orthonormalise all pending vectors now and apply them together
             block = [x]
             for y in pending:
                 y = _orthogonalize(y.copy(), vs + block)
                 block.append(y / np.linalg.norm(y))
             pending = block[1:]
-            acts = batch_op(block)
+            acts = yield block
             for y, a in zip(block[1:], acts[1:]):
                 prefetched[id(y)] = a
             ax = np.asarray(acts[0], dtype=complex)
         elif id(x) in prefetched:
             ax = np.asarray(prefetched.pop(id(x)), dtype=complex)
         else:
-            ax = np.asarray(op(x), dtype=complex)
+            ax = np.asarray((yield [x])[0], dtype=complex)
         if len(ax) != n or not np.isfinite(ax).all():
             raise SolverFailure("invalid operator action")
         change = 0.0 if previous is None else float(np.linalg.norm(ax - previous))
@@ -572,32 +610,49 @@ class ExcitationOperator:
                                   kinetic_options=self.kinetic_options)
 
 
+def kinetic_preconditioners(ops, *, nodes=8, floor=None, chunk=64, progress=None):
+    """kinetic_preconditioner of several operators on one coordinate set (e.g. one per momentum):
+    the coarse kinetic actions of every (operator, coordinate) pair share device calls of `chunk`
+    tangents."""
+    c = ops[0].coordinates
+    n = len(c)
+    model = ops[0].model
+    if model.kinetic == (0.0, 0.0):
+        raise ValueError("the kinetic preconditioner requires kinetic terms")
+    nodes = check_kinetic_nodes(nodes)
+    eye = np.eye(n, dtype=complex)
+    jobs = [(k, j) for k in range(len(ops)) for j in range(n)]
+    cols = [[None] * n for _ in ops]
+    for j0 in range(0, len(jobs), chunk):
+        part = jobs[j0:j0 + chunk]
+        fs = action_batch([ops[k].context for k, _ in part], [c.tangent_matrix(eye[j]) for _, j in part], model, nodes,
+                          _lib.EXC_KINETIC, krylov_options=PRECONDITIONER_KRYLOV)
+        for (k, j), f in zip(part, fs):
+            cols[k][j] = c.covector_coordinates(f)
+        if progress is not None:
+            progress(f"kinetic preconditioner: {min(j0 + chunk, len(jobs))}/{len(jobs)} coarse actions")
+    lower = max(model.kinetic) if floor is None else float(floor)
+    if not (math.isfinite(lower) and lower > 0):
+        raise ValueError("preconditioner floor must be positive")
+    out = []
+    for k in range(len(ops)):
+        m = np.stack(cols[k], 1)
+        if not np.isfinite(m).all():
+            raise SolverFailure("nonfinite kinetic preconditioner")
+        vals, vecs = np.linalg.eigh((m + m.conj().T) / 2)
+
+        def apply(r, vecs=vecs, vals=vals):
+            return vecs @ ((vecs.conj().T @ r) / np.maximum(lower, vals))
+        out.append(dict(apply=apply, matrix=m, values=vals, vectors=vecs,
+                        skew=float(np.linalg.norm(m - m.conj().T) / max(1.0, np.linalg.norm(m))), floor=lower,
+                        nodes=nodes))
+    return out
+
+
 def kinetic_preconditioner(op: ExcitationOperator, *, nodes=8, floor=None, chunk=64):
     """r -> sum_k v_k <v_k, r> / max(floor, kappa_k) from the coarse kinetic matrix on `nodes`
     theta nodes (src/excitation_eigsolve.jl:181-201); one batched kinetic action per chunk."""
-    c, ctx = op.coordinates, op.context
-    n = len(c)
-    if op.model.kinetic == (0.0, 0.0):
-        raise ValueError("the kinetic preconditioner requires kinetic terms")
-    nodes = check_kinetic_nodes(nodes)
-    cols = []
-    eye = np.eye(n, dtype=complex)
-    for j0 in range(0, n, chunk):
-        ws = [c.tangent_matrix(eye[j]) for j in range(j0, min(n, j0 + chunk))]
-        fs = action_batch(ctx, ws, op.model, nodes, _lib.EXC_KINETIC)
-        cols += [c.covector_coordinates(f) for f in fs]
-    k = np.stack(cols, 1)
-    if not np.isfinite(k).all():
-        raise SolverFailure("nonfinite kinetic preconditioner")
-    vals, vecs = np.linalg.eigh((k + k.conj().T) / 2)
-    lower = max(op.model.kinetic) if floor is None else float(floor)
-    if not (math.isfinite(lower) and lower > 0):
-        raise ValueError("preconditioner floor must be positive")
-
-    def apply(r):
-        return vecs @ ((vecs.conj().T @ r) / np.maximum(lower, vals))
-    return dict(apply=apply, matrix=k, values=vals, vectors=vecs,
-                skew=float(np.linalg.norm(k - k.conj().T) / max(1.0, np.linalg.norm(k))), floor=lower, nodes=nodes)
+    return kinetic_preconditioners([op], nodes=nodes, floor=floor, chunk=chunk)[0]
 
 
 def field_seed(op: ExcitationOperator):
@@ -657,12 +712,31 @@ def fresh_level(op, x):
                 residual=float(np.linalg.norm(h - value.real * x) / np.linalg.norm(x)), action=h)
 
 
-def solve_levels(coordinates, ctx, model, *, nev=1, kinetic_nodes=256, kinetic_options=None, preconditioner="kinetic",
-                 preconditioner_nodes=8, start="auto", davidson_options=None, validate=True, refined=None,
-                 refined_kinetic_nodes=None, validation_tol=1e-7, acceptance_residual=1e-5, callback=None,
-                 context_seconds=0.0):
-    """The iterative lowest-level solve (src/excitation_eigsolve.jl:302-401) without refinement
-    steps; validation uses the fresh standard and refined actions."""
+def _apply_many(ops, requests):
+    """Apply the operators to requested vectors in one device call: requests = [(operator index,
+    vector)]; returns the actions in order."""
+    if not requests:
+        return []
+    c = ops[0].coordinates
+    op0 = ops[0]
+    ws = [c.tangent_matrix(x) for _, x in requests]
+    fs = excitation_hamiltonian_action([ops[k].context for k, _ in requests], ws, model=op0.model,
+                                       kinetic_nodes=op0.kinetic_nodes, kinetic_options=op0.kinetic_options)
+    for k, _ in requests:
+        ops[k].calls += 1
+    return [c.covector_coordinates(f) for f in fs]
+
+
+def solve_levels_many(coordinates, ctxs, model, *, nev=1, kinetic_nodes=256, kinetic_options=None,
+                      preconditioner="kinetic", preconditioner_nodes=8, start="auto", davidson_options=None,
+                      validate=True, refined=None, refined_kinetic_nodes=None, validation_tol=1e-7,
+                      acceptance_residual=1e-5, callback=None, context_seconds=0.0, progress=None,
+                      preconditioner_chunk=64):
+    """The iterative lowest-level solve (src/excitation_eigsolve.jl:302-401, without refinement
+    steps) for one context per momentum on shared coordinates, the Davidson searches in lockstep:
+    every round applies the pending vectors of all searches in one device call.  Validation uses
+    the fresh standard and refined actions (both in one call each).  Returns one result per context.
+    progress(message), when given, is called after every device call."""
     n = len(coordinates)
     if not (isinstance(nev, (int, np.integer)) and 1 <= nev <= n):
         raise ValueError(f"nev={nev} must lie between 1 and the sector dimension {n}")
@@ -672,58 +746,98 @@ def solve_levels(coordinates, ctx, model, *, nev=1, kinetic_nodes=256, kinetic_o
         raise ValueError("preconditioner must be 'kinetic', 'none', or a function")
     if not (acceptance_residual > 0 and validation_tol > 0):
         raise ValueError("acceptance_residual and validation_tol must be positive")
-    op = ExcitationOperator(ctx, coordinates, model, kinetic_nodes=kinetic_nodes, kinetic_options=kinetic_options)
+    ops = [ExcitationOperator(ctx, coordinates, model, kinetic_nodes=kinetic_nodes, kinetic_options=kinetic_options)
+           for ctx in ctxs]
     tick = time.time()
-    pre = kinetic_preconditioner(op, nodes=preconditioner_nodes) if preconditioner == "kinetic" else None
-    apply = pre["apply"] if pre is not None else (None if preconditioner == "none" else preconditioner)
+    pres = (kinetic_preconditioners(ops, nodes=preconditioner_nodes, chunk=preconditioner_chunk, progress=progress)
+            if preconditioner == "kinetic" else [None] * len(ops))
+    applies = [p["apply"] if p is not None else (None if preconditioner == "none" else preconditioner) for p in pres]
     pre_seconds = time.time() - tick
-    starts, startkind = initial_vectors(op, start, nev, pre)
+    starts = [initial_vectors(op, start, nev, pre) for op, pre in zip(ops, pres)]
     tick = time.time()
-    res = davidson(op, starts, nev=nev, precondition=apply, options=options, callback=callback,
-                   batch_op=op.apply_many)
+    gens = [davidson_steps(sv, nev=nev, precondition=ap, options=options, callback=callback, blocks=True)
+            for (sv, _), ap in zip(starts, applies)]
+    results = [None] * len(ops)
+    pending = {}
+    for k, gen in enumerate(gens):
+        try:
+            pending[k] = next(gen)
+        except StopIteration as stop:
+            results[k] = stop.value
+    rounds = 0
+    while pending:
+        flat = [(k, x) for k in sorted(pending) for x in pending[k]]
+        acts = _apply_many(ops, flat)
+        rounds += 1
+        if progress is not None:
+            progress(f"Davidson round {rounds}: {len(flat)} actions for {len(pending)} searches")
+        pos = 0
+        for k in sorted(pending):
+            cnt = len(pending[k])
+            mine, pos = acts[pos:pos + cnt], pos + cnt
+            try:
+                pending[k] = gens[k].send(mine)
+            except StopIteration as stop:
+                results[k] = stop.value
+                del pending[k]
     solve_seconds = time.time() - tick
-    coefficients = [v.copy() for v in res["vectors"]]
     tick = time.time()
-    validation = None
-    refined_ctx = None
+    validation = [None] * len(ops)
+    refined_ctxs = [None] * len(ops)
     if validate:
-        rp = refined or refined_integration(ctx.integrator)
+        rp = refined or refined_integration(ctxs[0].integrator)
         rn = refined_kinetic_nodes if refined_kinetic_nodes is not None else (
-            None if op.kinetic_nodes is None else 2 * op.kinetic_nodes)
-        refined_ctx = ctx.with_settings(integrator=rp)
-        reference = op.with_context(refined_ctx, kinetic_nodes=rn)
-        validation = []
-        for x in coefficients:
-            std, fine = fresh_level(op, x), fresh_level(reference, x)
-            spread = abs(std["energy"] - fine["energy"])
-            imag = max(abs(std["imaginary"]), abs(fine["imaginary"]))
-            residual = max(std["residual"], fine["residual"])
-            validation.append(dict(standard_energy=std["energy"], standard_residual=std["residual"],
-                                   refined_energy=fine["energy"], refined_residual=fine["residual"],
-                                   bilinear_energy=None, refined_bilinear_energy=None, energy_spread=spread,
-                                   max_imaginary=imag, residual_target_met=residual <= validation_tol,
-                                   consistent=spread <= validation_tol and imag <= validation_tol,
-                                   accepted=spread <= validation_tol and imag <= validation_tol
-                                   and residual <= acceptance_residual))
+            None if ops[0].kinetic_nodes is None else 2 * ops[0].kinetic_nodes)
+        refined_ctxs = [ctx.with_settings(integrator=rp) for ctx in ctxs]
+        refs = [op.with_context(rc, kinetic_nodes=rn) for op, rc in zip(ops, refined_ctxs)]
+        flat = [(k, v) for k in range(len(ops)) for v in results[k]["vectors"]]
+        std = _apply_many(ops, flat)
+        fine = _apply_many(refs, flat)
+        for k in range(len(ops)):
+            validation[k] = []
+        for (k, x), hs, hf in zip(flat, std, fine):
+            levels = []
+            for h in (hs, hf):
+                value = np.vdot(x, h) / np.vdot(x, x)
+                levels.append((float(value.real), float(value.imag),
+                               float(np.linalg.norm(h - value.real * x) / np.linalg.norm(x))))
+            (es, ims, rs), (ef, imf, rf) = levels
+            spread, imag, residual = abs(es - ef), max(abs(ims), abs(imf)), max(rs, rf)
+            validation[k].append(dict(standard_energy=es, standard_residual=rs, refined_energy=ef, refined_residual=rf,
+                                      bilinear_energy=None, refined_bilinear_energy=None, energy_spread=spread,
+                                      max_imaginary=imag, residual_target_met=residual <= validation_tol,
+                                      consistent=spread <= validation_tol and imag <= validation_tol,
+                                      accepted=spread <= validation_tol and imag <= validation_tol
+                                      and residual <= acceptance_residual))
     validation_seconds = time.time() - tick
-    accepted = len(coefficients) == nev and (res["converged"] if validation is None
-                                             else all(v["accepted"] for v in validation))
-    vectors = [coordinates.original_tangent(coordinates.tangent_matrix(x)) for x in coefficients]
-    return dict(energies=list(res["energies"]), vectors=vectors, coefficients=coefficients,
-                residuals=list(res["residuals"]), converged=res["converged"], stop_reason=res["stop_reason"],
-                accepted=accepted, nev=nev, validation_tol=validation_tol, acceptance_residual=acceptance_residual,
-                numops=res["numops"], action_calls=op.calls, history=res["history"], momentum=ctx.momentum,
-                sector=coordinates.sector, parity=coordinates.parity, gauge=coordinates.gauge, dimension=n,
-                metric_rank=coordinates.metric_rank, start=startkind, davidson=options,
-                preconditioner=None if pre is None else dict(nodes=pre["nodes"], skew=pre["skew"], floor=pre["floor"],
-                                                              minimum=float(pre["values"].min())),
-                validation=validation,
-                validated_energies=None if validation is None else [v["refined_energy"] for v in validation],
-                integration=ctx.integrator, refined_integration=None if refined_ctx is None else refined_ctx.integrator,
-                kinetic_nodes=op.kinetic_nodes,
-                timings=dict(context=context_seconds, preconditioner=pre_seconds, solve=solve_seconds,
-                             validation=validation_seconds),
-                model=model.name, coordinates=coordinates)
+    out = []
+    for k, (op, ctx, res, pre) in enumerate(zip(ops, ctxs, results, pres)):
+        coefficients = [v.copy() for v in res["vectors"]]
+        val = validation[k] if validate else None
+        accepted = len(coefficients) == nev and (res["converged"] if val is None else all(v["accepted"] for v in val))
+        out.append(dict(
+            energies=list(res["energies"]), coefficients=coefficients,
+            vectors=[coordinates.original_tangent(coordinates.tangent_matrix(x)) for x in coefficients],
+            residuals=list(res["residuals"]), converged=res["converged"], stop_reason=res["stop_reason"],
+            accepted=accepted, nev=nev, validation_tol=validation_tol, acceptance_residual=acceptance_residual,
+            numops=res["numops"], action_calls=op.calls, history=res["history"], momentum=ctx.momentum,
+            sector=coordinates.sector, parity=coordinates.parity, gauge=coordinates.gauge, dimension=n,
+            metric_rank=coordinates.metric_rank, start=starts[k][1], davidson=options,
+            preconditioner=None if pre is None else dict(nodes=pre["nodes"], skew=pre["skew"], floor=pre["floor"],
+                                                          minimum=float(pre["values"].min())),
+            validation=val, validated_energies=None if val is None else [v["refined_energy"] for v in val],
+            integration=ctx.integrator,
+            refined_integration=None if refined_ctxs[k] is None else refined_ctxs[k].integrator,
+            kinetic_nodes=op.kinetic_nodes, device_rounds=rounds,
+            timings=dict(context=context_seconds, preconditioner=pre_seconds, solve=solve_seconds,
+                         validation=validation_seconds),
+            model=model.name, coordinates=coordinates))
+    return out
+
+
+def solve_levels(coordinates, ctx, model, **kw):
+    """The iterative lowest-level solve at one momentum (src/excitation_eigsolve.jl:302-401)."""
+    return solve_levels_many(coordinates, [ctx], model, **kw)[0]
 
 
 def excitation_eigsolve(s: State, p: float = 0.0, *, model: ExcitationModel | None = None, sector="auto",
@@ -744,13 +858,22 @@ def excitation_eigsolve(s: State, p: float = 0.0, *, model: ExcitationModel | No
 
 def iterative_dispersion(s: State, momenta, *, model=None, sector="auto", parity="auto", parity_tol=1e-12,
                          gauge="density", metric_tol=0.0, environment_tol=1e-10, max_bond=64, start="auto",
-                         strict=True, **kw):
-    """excitation_eigsolve at every momentum on shared coordinates, each momentum warm-started from
-    the previous eigenvectors (src/excitation_eigsolve.jl:453-476)."""
+                         strict=True, lockstep=False, **kw):
+    """excitation_eigsolve at every momentum on shared coordinates (src/excitation_eigsolve.jl:453-476).
+    Sequentially, each momentum warm-starts from the previous eigenvectors, as the reference does;
+    with lockstep=True every momentum starts from `start` and the Davidson searches of all momenta
+    share their device calls (one action per momentum per call)."""
     if s.D > max_bond:
         raise ValueError(f"excitation_eigsolve is limited to D<={max_bond}")
     model = model or ExcitationModel.of(s)
     c = TangentCoordinates(s, sector=sector, parity=parity, parity_tol=parity_tol, gauge=gauge, metric_tol=metric_tol)
+    if lockstep:
+        ctxs = [ExcitationContext(c.state, float(p), tol=environment_tol) for p in momenta]
+        results = solve_levels_many(c, ctxs, model, start=start, **kw)
+        if strict:
+            for r in results:
+                accepted_energies(r)
+        return results
     results = []
     for k, p in enumerate(momenta):
         ctx = ExcitationContext(c.state, float(p), tol=environment_tol)

@@ -204,3 +204,48 @@ def test_d32_eigsolve_matches_golden_level(rg, name):
     assert r["converged"] and r["dimension"] == e["dimension"]
     assert abs(r["energies"][0] - e["ritz_energy"]) <= e["ritz_tolerance"]
     assert abs(v["refined_energy"] - e["four_energies"][1]) <= e["ritz_tolerance"]
+
+
+def test_d4_lockstep_momenta_match_golden_levels(rg):
+    """The Davidson searches of p = 0 and 0.4 in lockstep (one device call per round applies one
+    action per momentum) give the golden levels of both momenta."""
+    note("case: iterative_dispersion lockstep D=4 odd p in (0, 0.4)")
+    from rcmps_gpu import DavidsonOptions, iterative_dispersion
+    levels = json.loads((GOLDEN / "excitation_references.json").read_text())["levels"]
+    s = d4_state(rg)
+    opts = DavidsonOptions(tol=1e-8, maxops=60, maxspace=8, retain=2, refresh=True)
+    rs = iterative_dispersion(s, [0.0, 0.4], nev=2, sector="odd", kinetic_nodes=32, davidson_options=opts,
+                              lockstep=True)
+    for r, p in zip(rs, (0.0, 0.4)):
+        gold = sorted((e["level"], e) for e in levels
+                      if e["state"] == "phi4_g1_D4" and e["sector"] == "odd" and e["momentum"] == p)
+        note(f"  p={p} energies {r['energies']} numops {r['numops']} rounds {r['device_rounds']}")
+        assert r["numops"] == gold[0][1]["numops"]
+        for (level, e), energy in zip(gold, r["energies"]):
+            assert abs(energy - e["ritz_energy"]) <= e["ritz_tolerance"]
+    assert rs[0]["device_rounds"] <= max(r["numops"] for r in rs)
+
+
+def test_repeated_actions_release_their_workspaces(rg):
+    """Every per-call device buffer of the action is released (a leak once ran a long dispersion
+    solve out of memory)."""
+    note("case: excitation action memory after repeated calls D=16")
+    import torch
+    from rcmps_gpu import IntegratorParameters
+    from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
+    R, K = random_arrays(16, 3, 0.3)
+    s = rg.State(R, K)
+    s.integrator_param = IntegratorParameters(n_quad=32, max_point=14.0)
+    ctx = ExcitationContext(s, 0.3)
+    rng = np.random.default_rng(4)
+    ws = [rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16)) for _ in range(4)]
+    model = ExcitationModel("phi4", 1.0)
+    action_batch(ctx, ws, model, 32)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(8):
+        action_batch(ctx, ws, model, 32)
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    note(f"  free memory change {(free1 - free0) / 2**20:.1f} MiB")
+    assert free0 - free1 < 64 * 2**20
