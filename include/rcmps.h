@@ -22,6 +22,7 @@
  *   rcmps_excitation_create   ExcitationContext(s, p)                     src/excitation_environments.jl:18-40
  *   rcmps_excitation_poisson  excitation_poisson, action_dual_poisson     src/excitation_environments.jl:75-85,
  *                                                                          src/excitation_action.jl:85-94
+ *   rcmps_excitation_element  excitation_element(ExcitationPair(ctx,U,W))  src/excitations.jl:98-113
  *   rcmps_excitation_action   excitation_hamiltonian_action(ctx, W)       src/excitation_action.jl:194-228
  *                             (excitation_kinetic_action_sum :135-179, excitation_observable_action :103-126);
  *                             rcmps_gpu HamiltonianAction                  gpu/src/rcmps_gpu/excitations/
@@ -201,6 +202,15 @@ int rcmps_excitation_action(rcmps_context* ctx, rcmps_excitation* ex, int n, con
 int rcmps_excitation_action_multi(rcmps_context* ctx, int n, rcmps_excitation* const* exs, const double* const* W,
                                   double* const* F, double g, const double* kinetic_coef, const rcmps_integrator* ip,
                                   int kinetic_nodes, int parts, rcmps_stats* stats);
+
+/* Connected matrix elements <Phi_p(U_i)|H - E0|Phi_p(W_i)> per 2 pi delta(0) of n tangent pairs (each with
+   its context; one state): the pair environments, the four-block :phi^4: transfer ODE and the kinetic
+   resolvent sum on kinetic_nodes theta nodes (excitation_element, src/excitations.jl:98-113, with
+   ExcitationPair src/excitation_environments.jl:105-127).  out (host): 2n doubles, the complex values.
+   The route the reference's validation calls "bilinear" (src/excitation_eigsolve.jl:273-279). */
+int rcmps_excitation_element(rcmps_context* ctx, int n, rcmps_excitation* const* exs, const double* const* U,
+                             const double* const* W, double g, const double* kinetic_coef, const rcmps_integrator* ip,
+                             int kinetic_nodes, int parts, double* out, rcmps_stats* stats);
 
 #ifdef __cplusplus
 }

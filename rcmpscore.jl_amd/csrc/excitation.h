@@ -34,6 +34,35 @@ struct RcQuarticArgs {
 };
 
 __global__ void rc_quartic_rhs(RcQuarticArgs g);
+
+// arguments of rc_pair_rhs: pair p's state at U + p ustride (matrix (channel c, block k) at (4 c + k)
+// Dp^2), its derivative at F + p fstride; per-pair data (Q - ip, Q + ip, v = -R^+ w, w, vbra^+, u^+) at
+// + p dstride and the overlap n at nov[2 p nstride]; epilogue fields as RcQuarticArgs
+struct RcPairArgs {
+    int Dp, npair, nch;
+    const double* U;
+    long long ustride;
+    double* F;
+    long long fstride;
+    const double *Q, *Qh, *R, *Rh;
+    const double *Qs, *Qb, *V, *W, *Vbh, *Uh;
+    long long dstride;
+    const double* nov;
+    long long nstride;
+    const double* coef;
+    double jac, jt;
+    int epi, stage;
+    const double* y;
+    const double* K;
+    long long sk;
+    RcStageCoef comb, comb3;
+    double* Unext;
+    const double* hp;
+    double h, atol, rtol;
+    double* partial;
+};
+
+__global__ void rc_pair_rhs(RcPairArgs g);
 __global__ void rc_sylvester_diag_c(int Dp, int D, int adjoint, double s_re, double s_im, const double* lam,
                                     const double* Y, double* Z);
 __global__ void rc_mdots(const double* V, long long sb, long long sv, int nk, const double* W, long long sw, int n,

@@ -303,3 +303,120 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
         g.partial[2 * slot + 1] = s3[0];
     }
 }
+
+// ---------------------------------------------------------------------------------------------
+// The connected four-block transfer of a tangent pair (u, w) (blocks 00, ket, bra, ket+bra) applied
+// to nch channels in one launch: output matrix (pair p, channel c, block k) =
+//   jac T(x_c)_k + c jt S(x_{c-1})_k
+// with T the four-block transfer (L on every block, -ip on the ket block, +ip on the bra block, the
+// ket insertion v x + w x R^+ from blocks 0 -> 1 and 2 -> 3, the bra insertion x vbra^+ + R x u^+ from
+// blocks 0 -> 2 and 1 -> 3, and w x0 u^+ - n x0 into block 3) and S the :phi^4: edge sources (left:
+// R x_k, w x_0 -> 1, w x_2 -> 3; right: x_k R^+, x_0 u^+ -> 2, x_1 u^+ -> 3).  With nch = 1, jac = 1 and
+// jt = 0 this is the transfer alone (the Krylov operator of the kinetic resolvent).  grid
+// ((Dp/16)^2, npair * nch * 4).
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rc_pair_rhs(RcPairArgs g) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T1[64 * 16];
+    __shared__ cplx T2[64 * 16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int Dp = g.Dp, nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
+    const int per = 4 * g.nch, mi = blockIdx.y, p = mi / per, r = mi % per, c = r / 4, k = r % 4;
+    const long long m = (long long)Dp * Dp;
+    const cplx* U = reinterpret_cast<const cplx*>(g.U) + (long long)p * g.ustride;
+    auto x = [&](int ch, int blk) { return U + (long long)(4 * ch + blk) * m; };
+    const long long dofs = (long long)p * g.dstride;
+    const cplx* Q = reinterpret_cast<const cplx*>(g.Q);
+    const cplx* Qh = reinterpret_cast<const cplx*>(g.Qh);
+    const cplx* R = reinterpret_cast<const cplx*>(g.R);
+    const cplx* Rh = reinterpret_cast<const cplx*>(g.Rh);
+    const cplx* Qs = reinterpret_cast<const cplx*>(g.Qs) + dofs;
+    const cplx* Qb = reinterpret_cast<const cplx*>(g.Qb) + dofs;
+    const cplx* v = reinterpret_cast<const cplx*>(g.V) + dofs;
+    const cplx* w = reinterpret_cast<const cplx*>(g.W) + dofs;
+    const cplx* vbh = reinterpret_cast<const cplx*>(g.Vbh) + dofs;
+    const cplx* uh = reinterpret_cast<const cplx*>(g.Uh) + dofs;
+    const double jac = g.coef ? g.coef[0] : g.jac;
+    const double jt = g.coef ? g.coef[1] : g.jt;
+    const double cj = c * jt;
+    cacc acc;
+    cacc_zero(acc);
+    if (jac != 0.0) {
+        const cplx* xk = x(c, k);
+        qprod(acc, k == 1 ? Qs : k == 2 ? Qb : Q, xk, Dp, bi, bj, wave, lane, jac);
+        qprod(acc, xk, Qh, Dp, bi, bj, wave, lane, jac);
+        if (k == 0) {
+            qcolumn(T1, xk, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
+        } else if (k == 1) {
+            qprod(acc, v, x(c, 0), Dp, bi, bj, wave, lane, jac);
+            qcolumn(T1, xk, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qcolumn(T2, x(c, 0), Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
+            qprod_t(acc, w, T2, Dp, bi, wave, lane, jac);
+        } else if (k == 2) {
+            qprod(acc, x(c, 0), vbh, Dp, bi, bj, wave, lane, jac);
+            qcolumn(T1, xk, Rh, x(c, 0), uh, Dp, bj, wave, lane, red);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
+        } else {
+            qprod(acc, v, x(c, 2), Dp, bi, bj, wave, lane, jac);
+            qprod(acc, x(c, 1), vbh, Dp, bi, bj, wave, lane, jac);
+            qcolumn(T1, xk, Rh, x(c, 1), uh, Dp, bj, wave, lane, red);
+            qcolumn(T2, x(c, 2), Rh, x(c, 0), uh, Dp, bj, wave, lane, red);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
+            qprod_t(acc, w, T2, Dp, bi, wave, lane, jac);
+        }
+    }
+    if (c >= 1 && cj != 0.0) {
+        const cplx* xp = x(c - 1, k);
+        qprod(acc, R, xp, Dp, bi, bj, wave, lane, cj);
+        qprod(acc, xp, Rh, Dp, bi, bj, wave, lane, cj);
+        if (k == 1) qprod(acc, w, x(c - 1, 0), Dp, bi, bj, wave, lane, cj);
+        if (k == 2) qprod(acc, x(c - 1, 0), uh, Dp, bi, bj, wave, lane, cj);
+        if (k == 3) {
+            qprod(acc, w, x(c - 1, 2), Dp, bi, bj, wave, lane, cj);
+            qprod(acc, x(c - 1, 1), uh, Dp, bi, bj, wave, lane, cj);
+        }
+    }
+    cplx val = qreduce(acc, red, wave, lane);
+    const long long local = (long long)(16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15);
+    if (k == 3 && jac != 0.0) {   // - n x0
+        const cplx nn = cmake(g.nov[2 * (long long)p * g.nstride], g.nov[2 * (long long)p * g.nstride + 1]);
+        val = csub(val, cscale(jac, cmul(nn, x(c, 0)[local])));
+    }
+    const long long e = (long long)p * g.fstride + (long long)r * m + local;
+    reinterpret_cast<cplx*>(g.F)[e] = val;
+    if (g.epi == 0) return;
+    // the ODE layout is contiguous (ustride = fstride = 4 nch Dp^2): e indexes y, K and Unext too
+    const cplx* Kc = reinterpret_cast<const cplx*>(g.K);
+    const cplx y0 = reinterpret_cast<const cplx*>(g.y)[e];
+    if (g.epi == 1) {
+        const double h = g.hp ? g.hp[0] : g.h;
+        cplx sum = cscale(g.comb.a[g.stage], val);
+        for (int j = 0; j < g.stage; ++j)
+            if (g.comb.a[j] != 0.0) sum = cadd(sum, cscale(g.comb.a[j], Kc[(long long)j * g.sk + e]));
+        reinterpret_cast<cplx*>(g.Unext)[e] = cadd(y0, cscale(h, sum));
+        return;
+    }
+    cplx r5 = cscale(g.comb.a[g.stage], val), r3 = cscale(g.comb3.a[g.stage], val);
+    for (int j = 0; j < g.stage; ++j) {
+        const cplx kj = Kc[(long long)j * g.sk + e];
+        if (g.comb.a[j] != 0.0) r5 = cadd(r5, cscale(g.comb.a[j], kj));
+        if (g.comb3.a[j] != 0.0) r3 = cadd(r3, cscale(g.comb3.a[j], kj));
+    }
+    const double sc = g.atol + g.rtol * fmax(cabsd(y0), cabsd(reinterpret_cast<const cplx*>(g.U)[e]));
+    double* s5 = reinterpret_cast<double*>(red);
+    double* s3 = s5 + 256;
+    s5[tid] = cabs2(r5) / (sc * sc);
+    s3[tid] = cabs2(r3) / (sc * sc);
+    __syncthreads();
+    for (int hh = 128; hh > 0; hh >>= 1) {
+        if (tid < hh) { s5[tid] += s5[tid + hh]; s3[tid] += s3[tid + hh]; }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long long slot = (long long)mi * nbt * nbt + blockIdx.x;
+        g.partial[2 * slot] = s5[0];
+        g.partial[2 * slot + 1] = s3[0];
+    }
+}

@@ -116,6 +116,10 @@ SIGNATURES = {
                                                      ctypes.POINTER(_P), ctypes.c_double,
                                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(Integrator),
                                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)]),
+    "rcmps_excitation_element": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                                ctypes.POINTER(_P), ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                                ctypes.POINTER(Integrator), ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(Stats)]),
 }
 EXC_KINETIC = 1
 EXC_POTENTIAL = 2

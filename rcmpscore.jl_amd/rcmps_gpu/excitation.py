@@ -9,10 +9,10 @@ on sector vectors of length D^2/2 on the host, as in the reference.
 Implemented: ExcitationModel (:phi4), ExcitationContext, excitation_hamiltonian_action,
 excitation_poisson / action_dual_poisson, TangentCoordinates, DavidsonOptions, davidson,
 ExcitationOperator, kinetic_preconditioner, field_seed, excitation_eigsolve,
-iterative_dispersion, dispersion_relation (iterative), relativistic_diagnostics.
-Validation computes the two fresh-action energies of the reference's four (standard and refined
-settings); the two bilinear energies (the connected-element route) are not computed here, so
-`accepted` rests on the fresh energies and residuals alone.
+iterative_dispersion, dispersion_relation (iterative), relativistic_diagnostics, excitation_element,
+tangent_metric.
+Validation computes the reference's four energies: fresh actions at the standard and refined
+settings, and the bilinear (connected-element) energies at both (excitation_element on the device).
 """
 from __future__ import annotations
 
@@ -193,6 +193,57 @@ def action_batch(ctx, ws, model: ExcitationModel, kinetic_nodes: int, parts: int
     _lib.check(st, "excitation action", stats)
     out = list(F.cpu().numpy())
     return (out, stats.as_dict()) if return_stats else out
+
+
+def excitation_element(ctx, us, ws, *, model: ExcitationModel | None = None, kinetic_nodes: int = 256, parts: int = 0,
+                       return_stats: bool = False):
+    """Connected matrix elements <Phi_p(u)|H - E0|Phi_p(w)> per 2 pi delta(0) for lists of tangent pairs
+    (excitation_element(ExcitationPair(ctx, u, w), model), src/excitations.jl:98-113) in one device
+    call; `ctx` is one context or one per pair.  The kinetic part uses the fixed theta grid."""
+    us = [np.asarray(u, dtype=np.complex128) for u in us]
+    ws = [np.asarray(w, dtype=np.complex128) for w in ws]
+    if len(us) != len(ws):
+        raise ValueError("one u per w is required")
+    ctxs = list(ctx) if isinstance(ctx, (list, tuple)) else [ctx] * len(ws)
+    if len(ctxs) != len(ws):
+        raise ValueError("one context per pair is required")
+    if not ws:
+        return np.zeros(0, complex)
+    s = ctxs[0].state
+    model = model or ExcitationModel.of(s)
+    ip0 = ctxs[0].integrator
+    for c in ctxs:
+        if c.state is not s or c.integrator != ip0:
+            raise ValueError("all contexts of one call must share the state and the integration settings")
+    for x in us + ws:
+        if x.shape != (s.D, s.D):
+            raise ValueError("tangent dimensions do not match the state")
+        if not np.isfinite(x).all():
+            raise ValueError("nonfinite tangent")
+    dev = _device(s)
+    Ut = torch.from_numpy(np.ascontiguousarray(np.stack(us))).to(dev)
+    Wt = torch.from_numpy(np.ascontiguousarray(np.stack(ws))).to(dev)
+    n = len(ws)
+    up = (ctypes.c_void_p * n)(*[Ut[i].data_ptr() for i in range(n)])
+    wp = (ctypes.c_void_p * n)(*[Wt[i].data_ptr() for i in range(n)])
+    hp = (ctypes.c_void_p * n)(*[c.handle for c in ctxs])
+    kc = (ctypes.c_double * 2)(*model.kinetic)
+    ip = _integrator(ip0)
+    out = (ctypes.c_double * (2 * n))()
+    stats = _lib.Stats()
+    apply_options(s.device, None, ACTION_KRYLOV)
+    torch.cuda.synchronize(dev)
+    st = _lib.load().rcmps_excitation_element(_lib.context(s.device), n, hp, up, wp, float(model.coupling), kc,
+                                              ctypes.byref(ip), int(check_kinetic_nodes(kinetic_nodes)), int(parts), out,
+                                              ctypes.byref(stats))
+    _lib.check(st, "excitation matrix element", stats)
+    vals = np.array(out[0::2]) + 1j * np.array(out[1::2])
+    return (vals, stats.as_dict()) if return_stats else vals
+
+
+def tangent_metric(s: State, u, w):
+    """The physical tangent inner product tr(w rho u^+) (src/excitation_environments.jl:88-92)."""
+    return complex(np.trace(np.asarray(w) @ s.rho @ np.asarray(u).conj().T))
 
 
 def excitation_hamiltonian_action(ctx: ExcitationContext, w, *, model: ExcitationModel | None = None,
@@ -727,6 +778,24 @@ def _apply_many(ops, requests):
     return [c.covector_coordinates(f) for f in fs]
 
 
+def _elements(ctxs, tangents, model, kinetic_nodes, kinetic_options):
+    """Diagonal connected elements <W|H - E0|W> of tangents (one context each), the kinetic grid fixed
+    or refined by node doubling (src/excitations.jl:82-95)."""
+    if kinetic_nodes is not None:
+        return excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=kinetic_nodes)
+    o = kinetic_options or KineticOptions()
+    pot = excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=2, parts=_lib.EXC_POTENTIAL)
+    n = o.minnodes
+    prev = excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=n, parts=_lib.EXC_KINETIC)
+    while 2 * n <= o.maxnodes:
+        n *= 2
+        cur = excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=n, parts=_lib.EXC_KINETIC)
+        if np.all(np.abs(cur - prev) <= o.atol + o.rtol * np.abs(cur)):
+            return cur + pot
+        prev = cur
+    raise SolverFailure(f"excitation kinetic quadrature did not converge by {n} nodes")
+
+
 def solve_levels_many(coordinates, ctxs, model, *, nev=1, kinetic_nodes=256, kinetic_options=None,
                       preconditioner="kinetic", preconditioner_nodes=8, start="auto", davidson_options=None,
                       validate=True, refined=None, refined_kinetic_nodes=None, validation_tol=1e-7,
@@ -734,8 +803,9 @@ def solve_levels_many(coordinates, ctxs, model, *, nev=1, kinetic_nodes=256, kin
                       preconditioner_chunk=64):
     """The iterative lowest-level solve (src/excitation_eigsolve.jl:302-401, without refinement
     steps) for one context per momentum on shared coordinates, the Davidson searches in lockstep:
-    every round applies the pending vectors of all searches in one device call.  Validation uses
-    the fresh standard and refined actions (both in one call each).  Returns one result per context.
+    every round applies the pending vectors of all searches in one device call.  Validation computes
+    the four energies of every level (fresh actions and bilinear elements, standard and refined
+    settings; each kind in one device call).  Returns one result per context.
     progress(message), when given, is called after every device call."""
     n = len(coordinates)
     if not (isinstance(nev, (int, np.integer)) and 1 <= nev <= n):
@@ -793,19 +863,31 @@ def solve_levels_many(coordinates, ctxs, model, *, nev=1, kinetic_nodes=256, kin
         flat = [(k, v) for k in range(len(ops)) for v in results[k]["vectors"]]
         std = _apply_many(ops, flat)
         fine = _apply_many(refs, flat)
+        # the bilinear energies: connected elements <W|H - E0|W> / tr(W rho W^+) at both settings
+        tang = [coordinates.tangent_matrix(x) for _, x in flat]
+        bil = _elements([ops[k].context for k, _ in flat], tang, model, ops[0].kinetic_nodes,
+                        ops[0].kinetic_options)
+        rbil = _elements([refs[k].context for k, _ in flat], tang, model, refs[0].kinetic_nodes,
+                         refs[0].kinetic_options)
         for k in range(len(ops)):
             validation[k] = []
-        for (k, x), hs, hf in zip(flat, std, fine):
+        for (k, x), hs, hf, wt, eb, erb in zip(flat, std, fine, tang, bil, rbil):
             levels = []
             for h in (hs, hf):
                 value = np.vdot(x, h) / np.vdot(x, x)
                 levels.append((float(value.real), float(value.imag),
                                float(np.linalg.norm(h - value.real * x) / np.linalg.norm(x))))
             (es, ims, rs), (ef, imf, rf) = levels
-            spread, imag, residual = abs(es - ef), max(abs(ims), abs(imf)), max(rs, rf)
+            norm = tangent_metric(coordinates.state, wt, wt)
+            vb, vrb = eb / norm, erb / norm
+            values = (es, ef, vb.real, vrb.real)
+            spread = max(values) - min(values)
+            imag = max(abs(ims), abs(imf), abs(vb.imag), abs(vrb.imag))
+            residual = max(rs, rf)
             validation[k].append(dict(standard_energy=es, standard_residual=rs, refined_energy=ef, refined_residual=rf,
-                                      bilinear_energy=None, refined_bilinear_energy=None, energy_spread=spread,
-                                      max_imaginary=imag, residual_target_met=residual <= validation_tol,
+                                      bilinear_energy=float(vb.real), refined_bilinear_energy=float(vrb.real),
+                                      energy_spread=spread, max_imaginary=imag,
+                                      residual_target_met=residual <= validation_tol,
                                       consistent=spread <= validation_tol and imag <= validation_tol,
                                       accepted=spread <= validation_tol and imag <= validation_tol
                                       and residual <= acceptance_residual))
@@ -825,7 +907,8 @@ def solve_levels_many(coordinates, ctxs, model, *, nev=1, kinetic_nodes=256, kin
             metric_rank=coordinates.metric_rank, start=starts[k][1], davidson=options,
             preconditioner=None if pre is None else dict(nodes=pre["nodes"], skew=pre["skew"], floor=pre["floor"],
                                                           minimum=float(pre["values"].min())),
-            validation=val, validated_energies=None if val is None else [v["refined_energy"] for v in val],
+            validation=val,
+            validated_energies=None if val is None else [v["refined_bilinear_energy"] for v in val],
             integration=ctx.integrator,
             refined_integration=None if refined_ctxs[k] is None else refined_ctxs[k].integrator,
             kinetic_nodes=op.kinetic_nodes, device_rounds=rounds,
@@ -836,7 +919,8 @@ def solve_levels_many(coordinates, ctxs, model, *, nev=1, kinetic_nodes=256, kin
 
 
 def solve_levels(coordinates, ctx, model, **kw):
-    """The iterative lowest-level solve at one momentum (src/excitation_eigsolve.jl:302-401)."""
+    """The iterative lowest-level solve at one momentum (src/excitation_eigsolve.jl:302-401); the
+    refinement steps (refine_steps) of the reference are not built."""
     return solve_levels_many(coordinates, [ctx], model, **kw)[0]
 
 

@@ -79,6 +79,35 @@ def test_action_parts_match_oracle(rg, d, seed, p):
     assert np.abs(full - (kin + pot)).max() <= 1e-12 * np.abs(full).max()
 
 
+@pytest.mark.parametrize("d,seed,p", [(2, 42, 0.7), (4, 7, 0.0)])
+def test_connected_element_matches_oracle(rg, d, seed, p):
+    """<Phi_p(u)|H - E0|Phi_p(w)> by the device pair environments, four-block ODE and kinetic Krylov
+    sum against the oracle's dense route (both integrate to 1e-12), and against <u, F(w)>."""
+    note(f"case: excitation element D={d} seed={seed} p={p}")
+    from rcmps_gpu import IntegratorParameters
+    from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch, excitation_element
+    R, K = random_arrays(d, seed, 0.5)
+    s = rg.State(R, K, steady_tol=1e-13)
+    s.integrator_param = IntegratorParameters(n_quad=64, max_point=18.0, atol=1e-12, rtol=1e-12)
+    ctx = ExcitationContext(s, p)
+    rng = np.random.default_rng(seed + 2)
+    u = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+    w = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+    model = ExcitationModel("phi4", 1.0)
+    (kin,) = excitation_element(ctx, [u], [w], model=model, kinetic_nodes=32, parts=1)
+    (pot,) = excitation_element(ctx, [u], [w], model=model, kinetic_nodes=32, parts=2)
+    xs = oracle_state(s, s.integrator_param)
+    pr = X.pair(X.context(xs, p), u, w)
+    terms = X.phi4_terms(1.0)
+    okin = X.excitation_kinetic_sum(pr, [t for t in terms if t[1].kinetic], 32)
+    opot = 1.0 * X.excitation_observable(pr, X.monomial_spec(4))
+    note(f"  kinetic {kin} vs {okin}; potential {pot} vs {opot}")
+    assert abs(kin - okin) <= 1e-10 * max(1.0, abs(okin))
+    assert abs(pot - opot) <= 1e-9 * max(1.0, abs(opot))
+    (f,) = action_batch(ctx, [w], model, 32)
+    assert abs(np.vdot(u, f) - (kin + pot)) <= 2e-8 * max(1.0, abs(kin + pot))
+
+
 def test_poisson_solves_match_oracle(rg):
     note("case: excitation Poisson solves D=4 p=0.7")
     from rcmps_gpu.excitation import ExcitationContext, action_dual_poisson, excitation_poisson
@@ -154,8 +183,12 @@ def test_d4_eigsolve_matches_golden_levels(rg, sector, p):
     assert r["converged"] and r["start"] == gold[0][1]["start"]
     for (level, e), energy, v in zip(gold, r["energies"], r["validation"]):
         assert abs(energy - e["ritz_energy"]) <= e["ritz_tolerance"], (level, energy, e["ritz_energy"])
-        assert abs(v["standard_energy"] - e["four_energies"][0]) <= e["tolerance"]
-        assert abs(v["refined_energy"] - e["four_energies"][1]) <= e["tolerance"]
+        four = (v["standard_energy"], v["refined_energy"], v["bilinear_energy"], v["refined_bilinear_energy"])
+        note(f"  level {level}: four energies {four} (golden {e['four_energies']})")
+        for mine, theirs in zip(four, e["four_energies"]):
+            assert abs(mine - theirs) <= e["tolerance"]
+        assert abs(r["validated_energies"][level - 1] - e["validated_energy"]) <= e["tolerance"]
+        assert v["accepted"] == e["accepted"]
 
 
 def test_d16_eigsolve_matches_golden_level(rg):
