@@ -9,7 +9,7 @@ from .excitation import (DavidsonOptions, ExcitationContext, ExcitationModel, Ex
                          TangentCoordinates, action_dual_poisson, davidson, dispersion_relation, excitation_eigsolve,
                          excitation_hamiltonian_action, excitation_parity, excitation_poisson, field_seed,
                          iterative_dispersion, kinetic_preconditioner, relativistic_diagnostics, excitation_element,
-                         tangent_metric)
+                         tangent_metric, excitation_matrix_element)
 from .jld2 import read_jld2
 from .io import StateRecord, rcmps_load, rcmps_store, read_portable, write_portable
 from .observables import (boundary_environment, fg_phi4, fg_phi4_batch, fg_phi4_raw, fg_phi4_rk,
@@ -30,5 +30,5 @@ __all__ = [
     "DavidsonOptions", "ExcitationContext", "ExcitationModel", "ExcitationOperator", "TangentCoordinates",
     "action_dual_poisson", "davidson", "dispersion_relation", "excitation_eigsolve", "excitation_hamiltonian_action",
     "excitation_parity", "excitation_poisson", "field_seed", "iterative_dispersion", "kinetic_preconditioner",
-    "relativistic_diagnostics", "excitation_element", "tangent_metric",
+    "relativistic_diagnostics", "excitation_element", "tangent_metric", "excitation_matrix_element",
 ]

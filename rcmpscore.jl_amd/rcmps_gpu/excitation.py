@@ -778,22 +778,59 @@ def _apply_many(ops, requests):
     return [c.covector_coordinates(f) for f in fs]
 
 
-def _elements(ctxs, tangents, model, kinetic_nodes, kinetic_options):
-    """Diagonal connected elements <W|H - E0|W> of tangents (one context each), the kinetic grid fixed
-    or refined by node doubling (src/excitations.jl:82-95)."""
+def _check_kinetic_options(o: KineticOptions):
+    """The quadrature controls check of src/excitations.jl:89-91."""
+    if not (o.minnodes >= 2 and o.maxnodes >= 2 * o.minnodes and np.isfinite(o.atol) and np.isfinite(o.rtol)
+            and o.atol > 0 and o.rtol >= 0):
+        raise ValueError("invalid kinetic quadrature controls")
+
+
+def _pair_elements(ctxs, us, ws, model, kinetic_nodes, kinetic_options):
+    """Connected elements <Phi_p(u)|H - E0|Phi_p(w)> of pairs (one context each), the kinetic grid fixed
+    or refined by node doubling until every pair's kinetic sum has converged
+    (excitation_element / excitation_kinetic, src/excitations.jl:88-119)."""
     if kinetic_nodes is not None:
-        return excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=kinetic_nodes)
+        return excitation_element(ctxs, us, ws, model=model, kinetic_nodes=kinetic_nodes)
     o = kinetic_options or KineticOptions()
-    pot = excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=2, parts=_lib.EXC_POTENTIAL)
+    _check_kinetic_options(o)
+    pot = excitation_element(ctxs, us, ws, model=model, kinetic_nodes=2, parts=_lib.EXC_POTENTIAL)
     n = o.minnodes
-    prev = excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=n, parts=_lib.EXC_KINETIC)
+    prev = excitation_element(ctxs, us, ws, model=model, kinetic_nodes=n, parts=_lib.EXC_KINETIC)
     while 2 * n <= o.maxnodes:
         n *= 2
-        cur = excitation_element(ctxs, tangents, tangents, model=model, kinetic_nodes=n, parts=_lib.EXC_KINETIC)
+        cur = excitation_element(ctxs, us, ws, model=model, kinetic_nodes=n, parts=_lib.EXC_KINETIC)
+        if not np.isfinite(cur).all():
+            raise SolverFailure("nonfinite excitation kinetic quadrature")
         if np.all(np.abs(cur - prev) <= o.atol + o.rtol * np.abs(cur)):
             return cur + pot
         prev = cur
     raise SolverFailure(f"excitation kinetic quadrature did not converge by {n} nodes")
+
+
+def _elements(ctxs, tangents, model, kinetic_nodes, kinetic_options):
+    """Diagonal connected elements <W|H - E0|W> of tangents (one context each)."""
+    return _pair_elements(ctxs, tangents, tangents, model, kinetic_nodes, kinetic_options)
+
+
+def excitation_matrix_element(s: State, p: float, u, w, *, model: ExcitationModel | None = None,
+                              environment_tol: float = 1e-10, kinetic_nodes=None,
+                              kinetic_options: KineticOptions | None = None):
+    """excitation_matrix_element(s, p, u, w) (src/excitations.jl:122-130): the connected element
+    <Phi_p(u)|H - E0|Phi_p(w)> per 2 pi delta(0) of horizontal tangents u, w (V = -R^+ W), a complex
+    sesquilinear form.  kinetic_nodes=None refines the theta grid by node doubling; an integer fixes
+    it.  `u` and `w` may be equal-length lists of tangents: one batched device call per grid, and an
+    array of elements is returned."""
+    many = isinstance(w, (list, tuple))
+    us, ws = (list(u), list(w)) if many else ([u], [w])
+    if kinetic_nodes is not None:
+        check_kinetic_nodes(kinetic_nodes)
+    ctx = ExcitationContext(s, p, tol=environment_tol)
+    try:
+        vals = _pair_elements([ctx] * len(ws), us, ws, model or ExcitationModel.of(s), kinetic_nodes,
+                              kinetic_options)
+    finally:
+        ctx.close()
+    return vals if many else complex(vals[0])
 
 
 def solve_levels_many(coordinates, ctxs, model, *, nev=1, kinetic_nodes=256, kinetic_options=None,

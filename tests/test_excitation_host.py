@@ -98,3 +98,18 @@ def test_relativistic_diagnostics_of_the_free_dispersion():
         relativistic_diagnostics(dict(momenta=np.array([-1.0, 1.0]), energies=np.ones((2, 1))))
     with pytest.raises(ValueError):
         relativistic_diagnostics(dict(momenta=np.array([0.0, 1.0]), energies=-np.ones((2, 1))))
+
+
+def test_kinetic_quadrature_controls_are_checked():
+    """The element's adaptive theta grid rejects controls with maxnodes < 2 minnodes, minnodes < 2 or
+    a non-positive atol (src/excitations.jl:89-91), and fixed grids below two nodes."""
+    from rcmps_gpu import KineticOptions
+    from rcmps_gpu.excitation import _check_kinetic_options, check_kinetic_nodes
+    _check_kinetic_options(KineticOptions())
+    for bad in (KineticOptions(minnodes=1, maxnodes=8), KineticOptions(minnodes=32, maxnodes=48),
+                KineticOptions(atol=0.0), KineticOptions(rtol=-1e-3), KineticOptions(atol=math.inf)):
+        with pytest.raises(ValueError):
+            _check_kinetic_options(bad)
+    assert check_kinetic_nodes(2) == 2
+    with pytest.raises(ValueError):
+        check_kinetic_nodes(1)

@@ -5,10 +5,13 @@ levels): on seeded D = 2 and D = 4 states at p = 0 and 0.7 the kinetic part (fix
 grid) and the :phi^4: part of F(W) agree with the oracle's dense-Schur and SciPy-DOP853 values within
 1e-9 relative to |F| (both integrate to 1e-12); the Poisson solves agree within 1e-11.  Closed forms:
 vacuum sqrt(1 + p^2) (1e-11), coherent D = 1 at g = 2 omega + 24 0.3^2 / omega (1e-8).  Batching:
-three tangents in one call equal three single calls to 1e-12.  Golden levels: excitation_eigsolve
-on the reference's D = 4 Z2 state (tests/golden) reproduces the recorded Davidson Ritz energies of
-both sectors at p = 0 and 0.4 within their tolerance 1e-11, and the refined fresh-action energy within
-the same tolerance.
+three tangents in one call equal three single calls to 1e-12.  Connected element: the device pair
+route agrees with the oracle's kinetic sum (1e-10) and :phi^4: ODE (1e-9), with <u, F(w)> (2e-8), is
+Hermitian between (u, w) and (w, u) (1e-9), and its adaptive grid agrees with the oracle's 512-node
+value (1e-9).  Golden levels: excitation_eigsolve on the reference's D = 4 Z2 state (tests/golden)
+reproduces the recorded Davidson Ritz energies of both sectors at p = 0 and 0.4, all four validation
+energies (standard and refined fresh actions, standard and refined bilinear elements), the validated
+energy and the acceptance, within their tolerance 1e-11.
 """
 import json
 import math
@@ -106,6 +109,35 @@ def test_connected_element_matches_oracle(rg, d, seed, p):
     assert abs(pot - opot) <= 1e-9 * max(1.0, abs(opot))
     (f,) = action_batch(ctx, [w], model, 32)
     assert abs(np.vdot(u, f) - (kin + pot)) <= 2e-8 * max(1.0, abs(kin + pot))
+
+
+def test_matrix_element_hermitian_and_adaptive(rg):
+    """excitation_matrix_element(s, p, u, w) with the adaptive theta grid: the sesquilinear form is
+    Hermitian (<u|H|w> = conj <w|H|u> from two independent pair integrations), its diagonal is real,
+    and it agrees with the oracle's element on a fixed fine grid to the quadrature tolerance."""
+    note("case: excitation_matrix_element D=4 seed=5 p=0.3 adaptive")
+    from rcmps_gpu import IntegratorParameters
+    from rcmps_gpu.excitation import ExcitationModel, excitation_matrix_element
+    R, K = random_arrays(4, 5, 0.5)
+    s = rg.State(R, K, steady_tol=1e-13)
+    s.integrator_param = IntegratorParameters(n_quad=64, max_point=18.0, atol=1e-12, rtol=1e-12)
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+    w = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+    model = ExcitationModel("phi4", 1.0)
+    uw, wu, uu = excitation_matrix_element(s, 0.3, [u, w, u], [w, u, u], model=model)
+    single = excitation_matrix_element(s, 0.3, u, w, model=model)
+    note(f"  <u|H|w> {uw}, conj <w|H|u> {np.conj(wu)}, <u|H|u> {uu}")
+    scale = max(1.0, abs(uw))
+    assert isinstance(single, complex) and abs(single - uw) <= 1e-12 * scale
+    assert abs(uw - np.conj(wu)) <= 1e-9 * scale
+    assert abs(uu.imag) <= 1e-9 * max(1.0, abs(uu))
+    xs = oracle_state(s, s.integrator_param)
+    pr = X.pair(X.context(xs, 0.3), u, w)
+    terms = X.phi4_terms(1.0)
+    ref = X.excitation_kinetic_sum(pr, [t for t in terms if t[1].kinetic], 512) \
+        + X.excitation_observable(pr, X.monomial_spec(4))
+    assert abs(uw - ref) <= 1e-9 * scale
 
 
 def test_poisson_solves_match_oracle(rg):
