@@ -325,6 +325,44 @@ __device__ __forceinline__ cplx reduce_split(const double* part, double* red, in
     return s;
 }
 
+// DOP853 combinations sum_j c_j k_j with the tableau as compile-time constants: zero terms
+// vanish and every index of the stage-derivative window k is static, so k stays in registers.
+// k_j sits at k[OFF + j].  The sum runs over j in increasing order from zero, as a running
+// accumulation would.
+enum RcRow { RC_ROW_A, RC_ROW_B, RC_ROW_E5, RC_ROW_E3, RC_ROW_D };
+
+template <RcRow W, int S>
+__device__ __forceinline__ constexpr double rc_coef(int j) {
+    if constexpr (W == RC_ROW_A) return RC_A[S][j];
+    if constexpr (W == RC_ROW_B) return j < RC_NSTAGES ? RC_B[j] : 0.0;
+    if constexpr (W == RC_ROW_E5) return j < RC_NSTAGES ? RC_E5[j] : 0.0;
+    if constexpr (W == RC_ROW_E3) return j < RC_NSTAGES ? RC_E3[j] : 0.0;
+    return RC_D[S][j];
+}
+
+template <RcRow W, int S, int N, int OFF, int J = 0>
+__device__ __forceinline__ cplx rc_comb(const cplx* k, cplx acc) {
+    if constexpr (J == N) {
+        return acc;
+    } else {
+        constexpr double c = rc_coef<W, S>(J);
+        if constexpr (c != 0.0) acc = cadd(acc, cscale(c, k[OFF + J]));
+        return rc_comb<W, S, N, OFF, J + 1>(k, acc);
+    }
+}
+
+// sum_{j < s} A[s][j] k_j for a uniform stage index s in 1..15, taken right after k_{s-1}
+// entered the window (k_j at k[16 - s + j])
+template <int S = 1>
+__device__ __forceinline__ cplx rc_stage_comb(int s, const cplx* k) {
+    if constexpr (S == RC_NSTAGES_EXT - 1) {
+        return rc_comb<RC_ROW_A, S, S, RC_NSTAGES_EXT - S>(k, cmake(0, 0));
+    } else {
+        if (s == S) return rc_comb<RC_ROW_A, S, S, RC_NSTAGES_EXT - S>(k, cmake(0, 0));
+        return rc_stage_comb<S + 1>(s, k);
+    }
+}
+
 }  // namespace
 
 template <int NB, bool SPLIT>
@@ -584,58 +622,24 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
     };
 
     // ------------------------------------------------------------------------------------
-    // integrator: a state machine over stage slots with a single stage call site.  Stage
-    // derivatives are folded into running combinations as they arrive, so no per-stage array
-    // is indexed at run time:  accA[s] = sum_j A[s][j] k_j (stage inputs), accB (solution),
-    // acc5/acc3 (error estimators), accD[r] (dense output).
+    // integrator: a state machine over stage slots with a single stage call site.  The stage
+    // derivatives of the attempt are kept per element in a window kst that shifts by one as each
+    // arrives (the newest at kst[15]); every combination (next stage input, solution, error
+    // estimators, dense output) is a sum with the tableau's constant coefficients, selected by
+    // the uniform slot, so no array is indexed at run time.
     // ------------------------------------------------------------------------------------
     const double a = p.a;
-    // accA[i] holds sum_j A[f + i][j] k_j for the stage f + i, where f (frame) is the last
-    // absorbed stage: the next stage input is always accA[1], so every index is static.
-    cplx accA[RC_NSTAGES_EXT], accB, acc5, acc3, accD[4];
+    cplx kst[RC_NSTAGES_EXT];
+#pragma unroll
+    for (int q = 0; q < RC_NSTAGES_EXT; ++q) kst[q] = cmake(0, 0);
     cplx y = cmake(0, 0), ynew = cmake(0, 0), k0 = cmake(0, 0), k12 = cmake(0, 0);
     __shared__ double s_trp[RC_NSTAGES_EXT];
     __shared__ double s_src[RC_NSTAGES_EXT];
     __shared__ double s_cheb[RC_SOURCE_SEGMENTS * (RC_SOURCE_DEGREE + 1)];
     for (int i = tid; i < RC_SOURCE_SEGMENTS * (RC_SOURCE_DEGREE + 1); i += 256) s_cheb[i] = p.cheb[i];
-    __syncthreads();
-    int frame = 0;
-
-    // row j of RC_COEF: accA coefficients in the frame of stage j, then B, E5, E3, D[0..3]
-    __shared__ double s_coef[RC_NSTAGES_EXT][RC_COEF_ROW];
     __shared__ double s_cnode[RC_NSTAGES_EXT];
-    for (int i = tid; i < RC_NSTAGES_EXT * RC_COEF_ROW; i += 256) (&s_coef[0][0])[i] = (&RC_COEF[0][0])[i];
     if (tid < RC_NSTAGES_EXT) s_cnode[tid] = RC_C[tid];
     __syncthreads();
-    auto absorb = [&](int j_in, cplx kj) {
-        const double* c = &s_coef[uni(j_in)][0];
-        double cv[22];
-#pragma unroll
-        for (int q = 0; q < 22; ++q) cv[q] = c[q];
-#pragma unroll
-        for (int i = 1; i < RC_NSTAGES_EXT; ++i) accA[i] = cadd(accA[i], cscale(cv[i - 1], kj));
-        accB = cadd(accB, cscale(cv[15], kj));
-        acc5 = cadd(acc5, cscale(cv[16], kj));
-        acc3 = cadd(acc3, cscale(cv[17], kj));
-#pragma unroll
-        for (int r = 0; r < 4; ++r) accD[r] = cadd(accD[r], cscale(cv[18 + r], kj));
-    };
-    // the frame advances to stage j: shift the pending combinations down by one
-    auto shift_to = [&](int j_in) {
-#pragma unroll
-        for (int i = 1; i < RC_NSTAGES_EXT - 1; ++i) accA[i] = accA[i + 1];
-        accA[RC_NSTAGES_EXT - 1] = cmake(0, 0);
-        frame = uni(j_in);
-    };
-    auto reset_acc = [&]() {
-#pragma unroll
-        for (int s = 0; s < RC_NSTAGES_EXT; ++s) accA[s] = cmake(0, 0);
-        accB = acc5 = acc3 = cmake(0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) accD[r] = cmake(0, 0);
-        frame = 0;
-        absorb(0, k0);
-    };
 
     double ysc = 0.0, ysc_new = 0.0;
     double t = -a, h = fmin(p.h0, 2.0 * a), t_new = t, endpoint = 0.0, factor = 1.0;
@@ -658,16 +662,19 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
         if (!(alive = stage(slot, ev, ts, src, kn, trn))) break;
         if (tid == 0) s_trp[slot] = trn;
         int next = -1;   // -1: start a new step attempt
+#pragma unroll
+        for (int q = 0; q < RC_NSTAGES_EXT - 1; ++q) kst[q] = kst[q + 1];
+        kst[RC_NSTAGES_EXT - 1] = kn;
         if (slot == 0) {
             k0 = kn;
         } else if (slot < RC_NSTAGES - 1) {
-            absorb(slot, kn);
-            shift_to(slot);
             next = slot + 1;
         } else if (slot == RC_NSTAGES - 1) {
-            absorb(slot, kn);
-            shift_to(slot);
             lap(7);
+            // k_j at kst[4 + j] after k_11
+            const cplx accB = rc_comb<RC_ROW_B, 0, RC_NSTAGES, 4>(kst, cmake(0, 0));
+            const cplx acc5 = rc_comb<RC_ROW_E5, 0, RC_NSTAGES, 4>(kst, cmake(0, 0));
+            const cplx acc3 = rc_comb<RC_ROW_E3, 0, RC_NSTAGES, 4>(kst, cmake(0, 0));
             ynew = cadd(y, cscale(h, accB));
             double e5p = 0.0, e3p = 0.0;
             if (real_el) {
@@ -748,8 +755,6 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             }
         } else if (slot == RC_NSTAGES) {
             k12 = kn;
-            absorb(slot, kn);
-            shift_to(slot);
             if (next_point < p.npoints && ld_const(p.points, next_point) <= t_new) {
                 ++dense;
                 next = RC_NSTAGES + 1;
@@ -757,17 +762,17 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
                 next = -2;   // finish the accepted step
             }
         } else if (slot < RC_NSTAGES_EXT - 1) {
-            absorb(slot, kn);
-            shift_to(slot);
             next = slot + 1;
         } else {
-            absorb(slot, kn);
             const cplx d0 = csub(ynew, y);
             const cplx d1 = csub(cscale(h, k0), d0);
             const cplx d2 = csub(cscale(2.0, d0), cscale(h, cadd(k12, k0)));
+            // k_j at kst[j] after k_15
             cplx dd[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) dd[r] = cscale(h, accD[r]);
+            dd[0] = cscale(h, rc_comb<RC_ROW_D, 0, RC_NSTAGES_EXT, 0>(kst, cmake(0, 0)));
+            dd[1] = cscale(h, rc_comb<RC_ROW_D, 1, RC_NSTAGES_EXT, 0>(kst, cmake(0, 0)));
+            dd[2] = cscale(h, rc_comb<RC_ROW_D, 2, RC_NSTAGES_EXT, 0>(kst, cmake(0, 0)));
+            dd[3] = cscale(h, rc_comb<RC_ROW_D, 3, RC_NSTAGES_EXT, 0>(kst, cmake(0, 0)));
             while (next_point < p.npoints && ld_const(p.points, next_point) <= t_new) {
                 const double pt = ld_const(p.points, next_point);
                 cplx v;
@@ -808,7 +813,7 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             if (!(h > 10.0 * fabs(nextafter(t, INFINITY) - t))) { status = RC_ST_STEP_UNDERFLOW; break; }
             if (attempts >= p.max_steps) { status = RC_ST_MAX_STEPS; break; }
             ++attempts;
-            reset_acc();
+            kst[RC_NSTAGES_EXT - 1] = k0;   // k_0 of the attempt enters the window
             // source values at every stage time of this attempt (stages 1..15), one lane each
             if (tid >= 1 && tid < RC_NSTAGES_EXT) s_src[tid] = rc_source(t + s_cnode[tid] * h, s_cheb);
             __syncthreads();
@@ -822,7 +827,7 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             Yin = ynew;
             ts = t + h;
         } else {
-            Yin = cadd(y, cscale(h, accA[1]));
+            Yin = cadd(y, cscale(h, rc_stage_comb(slot, kst)));
             ts = t + RC_C[slot] * h;
         }
     }
