@@ -256,7 +256,8 @@ __device__ __forceinline__ void block_scales(const double* m, int* S, double* sc
 }
 
 // Thread t holds four consecutive k of one row (row t / 16, k = 4 (t % 16) + q) of a row block,
-// or of one column (column t % 16, k = 4 (t / 16) + q) of a column block; zero beyond Dp.
+// or of one column (column t / 16, k = 4 (t % 16) + q) of a column block; zero beyond Dp.  The 16
+// lanes of one row or column are consecutive, so its largest entry is a 16-lane reduction.
 template <int NB, bool COH>
 __device__ __forceinline__ void fetch_row4(cplx* v, const double* g, uint32_t gbytes, int b) {
     constexpr int Dp = 16 * NB;
@@ -275,7 +276,7 @@ __device__ __forceinline__ void fetch_row4(cplx* v, const double* g, uint32_t gb
 template <int NB, bool COH>
 __device__ __forceinline__ void fetch_col4(cplx* v, const double* g, uint32_t gbytes, int c) {
     constexpr int Dp = 16 * NB;
-    const int t = threadIdx.x, j = t & 15, k0 = 4 * (t >> 4);
+    const int t = threadIdx.x, j = t >> 4, k0 = 4 * (t & 15);
     if (COH) {
         const __amdgpu_buffer_rsrc_t r = rc_rsrc(g, gbytes);
 #pragma unroll
@@ -294,8 +295,29 @@ __device__ __forceinline__ void store_row4(unsigned char* panel, const cplx* v, 
 }
 template <int NB>
 __device__ __forceinline__ void store_col4(unsigned char* panel, const cplx* v, int S) {
-    const int t = threadIdx.x, k0 = 4 * (t >> 4);
-    if (k0 < 16 * NB) rc_store_digits4(panel, t & 15, k0, v, ldexp(1.0, S));
+    const int t = threadIdx.x, k0 = 4 * (t & 15);
+    if (k0 < 16 * NB) rc_store_digits4(panel, t >> 4, k0, v, ldexp(1.0, S));
+}
+
+// biased FP64 exponent of the largest |re|, |im| of v[0..3] (0 for zeros and subnormals)
+__device__ __forceinline__ int max_exp4(const cplx* v) {
+    int e = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        e = max(e, (int)((uint32_t)__double2hiint(v[q].re) >> 20) & 0x7ff);
+        e = max(e, (int)((uint32_t)__double2hiint(v[q].im) >> 20) & 0x7ff);
+    }
+    return e;
+}
+// scale of this thread's row (or column) of a row (column) block: the maximum over its 16 lanes.
+// The lane holding k = 0 writes it to sg[row or column].
+__device__ __forceinline__ int line_scale(const cplx* v, int* sg) {
+    int e = max_exp4(v);
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+    const int S = rc_split_scale_e(e);
+    if ((threadIdx.x & 15) == 0) sg[threadIdx.x >> 4] = S;
+    return S;
 }
 
 // one constant panel (row block b or column block c of a fixed matrix) at kernel start
@@ -415,6 +437,10 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
 
     // constant blocks
     int sK0 = 0, sK1 = 0, sK2 = 0, sK3 = 0, sK4 = 0;   // split-precision panel scales
+    // per-row / per-column scales of the stage input's row block [0, 16) and column block
+    // [16, 32), and of the P block (split precision)
+    int* sgX = reinterpret_cast<int*>(scratch + 16);
+    int* sgP = sgX + 32;
     if constexpr (SPLIT) {
         // zero every digit plane once: k beyond Dp stays zero
         for (int i = tid; i < 8 * RC_PANEL_BYTES / 16; i += 256) reinterpret_cast<v4i*>(smem_raw)[i] = v4i{0, 0, 0, 0};
@@ -483,31 +509,37 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
         lap(0);
         const double* Ych = S.Y + ((size_t)si * S.nlive + ch) * Dp * Dp * 2;
         cplx xel = cmake(0, 0);   // own element of the stage input (split path)
-        int sXr = 0, sXc = 0;
         if constexpr (SPLIT) {
             cplx vr[4], vc[4];
             fetch_row4<NB, true>(vr, Ych, mat_bytes, b);
             fetch_col4<NB, true>(vc, Ych, mat_bytes, c);
             if (tk.trace_kind == 1) xel = ld_coh(rY, (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes + el_off);
-            const double m[2] = {max4(vr), max4(vc)};
-            int sc[2];
-            block_scales<2>(m, sc, scratch);
-            sXr = sc[0];
-            sXc = sc[1];
-            store_row4<NB>(pXr, vr, sXr);
-            store_col4<NB>(pXc, vc, sXc);
+            store_row4<NB>(pXr, vr, line_scale(vr, sgX));
+            store_col4<NB>(pXc, vc, line_scale(vc, sgX + 16));
         } else {
             load_row_col_coh<NB>(Xr, Xc, Ych, mat_bytes, b, c);
         }
         __syncthreads();
+        // per-element product scales: this lane's output rows 4 (lane/16) + q and column lane%16
+        int sXr[4], sXc = 0;
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sXr[q] = sgX[4 * (lane >> 4) + q];
+            sXc = sgX[16 + (lane & 15)];
+        }
         lap(1);
         // level 1
         cacc acc;
         double part[4] = {0.0, 0.0, 0.0, 0.0};
         cplx Pe;
         if constexpr (SPLIT) {
-            if (!adj) rc_split_product(part, pK0, spa, pXc, spb, -(sK0 + sXc), lane);
-            else rc_split_product(part, pXr, spa, pK0, spb, -(sXr + sK0), lane);
+            if (!adj) {
+                const int s2[4] = {-(sK0 + sXc), -(sK0 + sXc), -(sK0 + sXc), -(sK0 + sXc)};
+                rc_split_product(part, pK0, spa, pXc, spb, s2, lane);
+            } else {
+                const int s2[4] = {-(sXr[0] + sK0), -(sXr[1] + sK0), -(sXr[2] + sK0), -(sXr[3] + sK0)};
+                rc_split_product(part, pXr, spa, pK0, spb, s2, lane);
+            }
             Pe = reduce_split(part, redd, wave, lane);
         } else {
             cacc_zero(acc);
@@ -521,7 +553,8 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             cplx Ee;
             if constexpr (SPLIT) {
                 double pe[4] = {0.0, 0.0, 0.0, 0.0};
-                rc_split_product(pe, pXr, spa, pK4, spb, -(sXr + sK4), lane);
+                const int s2[4] = {-(sXr[0] + sK4), -(sXr[1] + sK4), -(sXr[2] + sK4), -(sXr[3] + sK4)};
+                rc_split_product(pe, pXr, spa, pK4, spb, s2, lane);
                 Ee = reduce_split(pe, redd, wave, lane);
             } else {
                 cacc_zero(acc);
@@ -542,8 +575,10 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
         // level 2a
         if constexpr (SPLIT) {
             part[0] = part[1] = part[2] = part[3] = 0.0;
-            rc_split_product(part, pK1, spa, pXc, spb, -(sK1 + sXc), lane);
-            rc_split_product(part, pXr, spa, pK2, spb, -(sXr + sK2), lane);
+            const int s2a[4] = {-(sK1 + sXc), -(sK1 + sXc), -(sK1 + sXc), -(sK1 + sXc)};
+            const int s2b[4] = {-(sXr[0] + sK2), -(sXr[1] + sK2), -(sXr[2] + sK2), -(sXr[3] + sK2)};
+            rc_split_product(part, pK1, spa, pXc, spb, s2a, lane);
+            rc_split_product(part, pXr, spa, pK2, spb, s2b, lane);
         } else {
             cacc_zero(acc);
             tile_product<NB>(acc, Kq, Xc, wave, lane);
@@ -586,19 +621,23 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             }
             ins = cscale(tk.edge_coef, cadd(s1, cconj(s2)));
         }
-        int sPb = 0;
         if constexpr (SPLIT) {
-            const double m = max4(vp);
-            block_scales<1>(&m, &sPb, scratch);
-            if (!adj) store_row4<NB>(pPb, vp, sPb);
-            else store_col4<NB>(pPb, vp, sPb);
+            if (!adj) store_row4<NB>(pPb, vp, line_scale(vp, sgP));
+            else store_col4<NB>(pPb, vp, line_scale(vp, sgP));
         }
         __syncthreads();
         lap(5);
         cplx Le;
         if constexpr (SPLIT) {
-            if (!adj) rc_split_product(part, pPb, spa, pK3, spb, -(sPb + sK3), lane);
-            else rc_split_product(part, pK3, spa, pPb, spb, -(sK3 + sPb), lane);
+            if (!adj) {
+                const int s2[4] = {-(sgP[4 * (lane >> 4)] + sK3), -(sgP[4 * (lane >> 4) + 1] + sK3),
+                                   -(sgP[4 * (lane >> 4) + 2] + sK3), -(sgP[4 * (lane >> 4) + 3] + sK3)};
+                rc_split_product(part, pPb, spa, pK3, spb, s2, lane);
+            } else {
+                const int sc = -(sK3 + sgP[lane & 15]);
+                const int s2[4] = {sc, sc, sc, sc};
+                rc_split_product(part, pK3, spa, pPb, spb, s2, lane);
+            }
             Le = reduce_split(part, redd, wave, lane);
         } else {
             if (!adj) tile_product<NB>(acc, Pb, Kp, wave, lane);
