@@ -153,6 +153,22 @@ int rcmps_fg_phi4_rk(rcmps_context* ctx, const rcmps_state* st, double g, const 
    1e-12, 16, 512).  RCMPS_ERR_ARGUMENT for invalid controls (ArgumentError in the reference). */
 int rcmps_set_kinetic_options(rcmps_context* ctx, const rcmps_kinetic_options* opt);
 
+/* Tile variant of this context's batched complex products: -1 (default) lets the autotuner pick
+   per (padded size, batch class), 0..3 force one (0: one 16x16 tile per four-wave workgroup with
+   the k-steps split, 1: four tiles per workgroup, one per wave, 2: one tile per one-wave
+   workgroup, 3: a 32x32 block per workgroup from operand panels staged in LDS).  Every variant
+   returns the same bits.  No counterpart in the reference, whose
+   products are library GEMMs; the tuner is the batching path's tile-shape choice.
+   RCMPS_ERR_ARGUMENT for another value. */
+int rcmps_gemm_variant(rcmps_context* ctx, int variant);
+
+/* The autotuner's table: for up to `cap` entries, the padded size, batch class c (batches of
+   2^(c-1) < n <= 2^c), the chosen variant and the measured microseconds per launch of each of
+   the four variants (us[4 i + v]); *count receives the number of entries.  No counterpart in
+   the reference. */
+int rcmps_gemm_tuning(rcmps_context* ctx, int cap, int* padded_size, int* batch_class, int* variant, double* us,
+                      int* count);
+
 /* kinetic1 (which = 1), kinetic2 (which = 2) or both (which = 3) on the infinite line by the Krylov
    backend: kinetic_resolvent (src/kinetic.jl:77-124) over kinetic_components
    (src/kinetic_krylov.jl:156-189).  raw_cov, raw_env (device, may be null) are the covector and

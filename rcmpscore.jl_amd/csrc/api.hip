@@ -10,6 +10,7 @@
 #include <complex>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -182,6 +183,15 @@ struct rcmps_context {
     // infinite-line kinetic backend: controls (KineticOptions / KineticKrylovOptions defaults) and workspace
     rcmps_kinetic_options kopt{32, 512, 1e-11, 1e-10, 1e-12, 1e-12, 16, 512};
     KineticWork kin;
+    // batched-GEMM tile variant per (padded size, batch class ceil(log2 nbatch)): measured on first
+    // use unless forced (gemm_force >= 0)
+    struct GemmTune {
+        int variant = 0;
+        double us[4] = {0.0, 0.0, 0.0, 0.0};
+    };
+    int gemm_force = -1;
+    std::map<std::pair<int, int>, GemmTune> gemm_tuned;
+    DevBuf tune_buf;
 };
 
 struct rcmps_state {
@@ -194,6 +204,80 @@ struct rcmps_state {
 };
 
 namespace {
+
+// ------------------------------------------------------------------------------------------
+// batched-GEMM tile variants and their autotuner.  Variant 0: one 16x16 output tile per
+// workgroup of four waves splitting the k-steps (LDS reduction); 1: four tiles per workgroup, one
+// per wave; 2: one tile per single-wave workgroup; 3: a 32x32 block per workgroup with both
+// operand panels staged in LDS.  All of them produce the same bits (the per-wave variants keep
+// the four k-step classes apart and sum them in variant 0's order), so the choice is purely a
+// speed decision: the first product of a (padded size, batch class) on a context times each
+// variant on operands of that shape and keeps the fastest.
+// ------------------------------------------------------------------------------------------
+constexpr int kGemmVariants = 4;
+
+size_t gemm_lds_bytes(int Dp) { return (size_t)2 * 32 * (Dp + 1) * 16; }
+
+int gemm_class(int nbatch) {
+    int c = 0;
+    while ((1 << c) < nbatch && c < 30) ++c;
+    return c;
+}
+
+void launch_zgemm_variant(hipStream_t stream, const GemmArgs& g, int nbatch, int v) {
+    const int nt = (g.Dp / 16) * (g.Dp / 16);
+    if (v == 1) hipLaunchKernelGGL(rc_zgemm_batched_w<4>, dim3(ceil_div(nt, 4), nbatch), dim3(256), 0, stream, g);
+    else if (v == 2) hipLaunchKernelGGL(rc_zgemm_batched_w<1>, dim3(nt, nbatch), dim3(64), 0, stream, g);
+    else if (v == 3) {
+        const int nb2 = (g.Dp / 16 + 1) / 2;
+        hipLaunchKernelGGL(rc_zgemm_batched_lds, dim3(nb2 * nb2, nbatch), dim3(256), gemm_lds_bytes(g.Dp), stream, g);
+    } else hipLaunchKernelGGL(rc_zgemm_batched, dim3(nt, nbatch), dim3(256), 0, stream, g);
+}
+
+int gemm_variant(rcmps_context* ctx, int Dp, int nbatch) {
+    if (ctx->gemm_force >= 0) return ctx->gemm_force;
+    const auto key = std::make_pair(Dp, gemm_class(nbatch));
+    auto it = ctx->gemm_tuned.find(key);
+    if (it != ctx->gemm_tuned.end()) return it->second.variant;
+    // a stream under graph capture takes the default without measuring
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return 0;
+    const int nb_t = std::min(1 << key.second, 1024);
+    const size_t mat = (size_t)Dp * Dp * 2;   // doubles per matrix
+    ck(ctx->tune_buf.ensure(3 * (size_t)nb_t * mat * 8));
+    double* buf = ctx->tune_buf.d();
+    ck(hipMemsetAsync(buf, 0, 3 * (size_t)nb_t * mat * 8, ctx->stream));
+    GemmArgs g{};
+    g.Dp = Dp; g.D = Dp; g.nbatch = nb_t;
+    g.A = buf; g.B = buf + nb_t * mat; g.C = buf + 2 * nb_t * mat;
+    g.strideA = g.strideB = g.strideC = (long long)Dp * Dp;
+    hipEvent_t e0, e1;
+    ck(hipEventCreate(&e0));
+    ck(hipEventCreate(&e1));
+    rcmps_context::GemmTune t;
+    constexpr int reps = 5;
+    for (int v = 0; v < kGemmVariants; ++v) {
+        launch_zgemm_variant(ctx->stream, g, nb_t, v);
+        ck(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; ++r) launch_zgemm_variant(ctx->stream, g, nb_t, v);
+        ck(hipEventRecord(e1, ctx->stream));
+        ck(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        ck(hipEventElapsedTime(&ms, e0, e1));
+        t.us[v] = 1e3 * ms / reps;
+        if (t.us[v] < t.us[t.variant]) t.variant = v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ck(hipGetLastError());
+    ctx->gemm_tuned[key] = t;
+    return t.variant;
+}
+
+void launch_zgemm(rcmps_context* ctx, const GemmArgs& g, int nbatch) {
+    launch_zgemm_variant(ctx->stream, g, nbatch, gemm_variant(ctx, g.Dp, nbatch));
+    ck(hipGetLastError());
+}
 
 // ------------------------------------------------------------------------------------------
 // device helpers on Dp x Dp matrices
@@ -211,13 +295,11 @@ struct Dev {
         g.A = A; g.B = B; g.C = C;
         g.beta = beta;
         g.alpha_scale = alpha;
-        hipLaunchKernelGGL(rc_zgemm_batched, dim3((Dp / 16) * (Dp / 16), 1), dim3(256), 0, ctx->stream, g);
-        ck(hipGetLastError());
+        launch_zgemm(ctx, g, 1);
     }
     void batched(GemmArgs g, int nbatch) {
         g.Dp = Dp; g.D = D; g.nbatch = nbatch;
-        hipLaunchKernelGGL(rc_zgemm_batched, dim3((Dp / 16) * (Dp / 16), nbatch), dim3(256), 0, ctx->stream, g);
-        ck(hipGetLastError());
+        launch_zgemm(ctx, g, nbatch);
     }
     // C = beta C + scale * sum_b alpha_b op(A_b) op(B_b)
     void reduce(GemmArgs g, int nbatch, double* C, double scale, double beta) {
@@ -562,6 +644,9 @@ int rcmps_context_create(int device, rcmps_context** out) {
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
             ctx->clock_mhz = khz / 1000.0;
         ck(ctx->abort_word.ensure(256));
+        // the LDS-staged product variant takes up to 66 KB of dynamic LDS at Dp = 64
+        ck(hipFuncSetAttribute(reinterpret_cast<const void*>(rc_zgemm_batched_lds),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)gemm_lds_bytes(64)));
     });
     if (rc) {
         delete ctx;
@@ -579,7 +664,7 @@ void rcmps_context_destroy(rcmps_context* ctx) {
                      &ctx->traj_points, &ctx->abort_word, &ctx->nodes_w, &ctx->nodes_j,
                      &ctx->tmpM, &ctx->partial, &ctx->scratch, &ctx->small, &ctx->kry_V, &ctx->kry_w,
                      &ctx->kry_t1, &ctx->kry_t2, &ctx->kry_x, &ctx->kry_b, &ctx->kry_z, &ctx->kry_h,
-                     &ctx->kry_tmp};
+                     &ctx->kry_tmp, &ctx->tune_buf};
     for (auto* b : all) b->release();
     for (auto& b : ctx->tb) b->release();
     ctx->kin.release();
@@ -1729,6 +1814,30 @@ int rcmps_fg_phi4_batch(rcmps_context* ctx, int n, const rcmps_state* const* sta
         reqs[i].S = stats ? stats + i : &local[i];
     }
     return fg_phi4_batch_impl(ctx, reqs, ip, metric_reg, flags);
+}
+
+int rcmps_gemm_variant(rcmps_context* ctx, int variant) {
+    if (!ctx || variant < -1 || variant >= kGemmVariants) return RCMPS_ERR_ARGUMENT;
+    ctx->gemm_force = variant;
+    return RCMPS_OK;
+}
+
+int rcmps_gemm_tuning(rcmps_context* ctx, int cap, int* padded_size, int* batch_class, int* variant, double* us,
+                      int* count) {
+    if (!ctx || !count || cap < 0 || (cap > 0 && (!padded_size || !batch_class || !variant || !us)))
+        return RCMPS_ERR_ARGUMENT;
+    int n = 0;
+    for (const auto& kv : ctx->gemm_tuned) {
+        if (n < cap) {
+            padded_size[n] = kv.first.first;
+            batch_class[n] = kv.first.second;
+            variant[n] = kv.second.variant;
+            for (int v = 0; v < kGemmVariants; ++v) us[kGemmVariants * n + v] = kv.second.us[v];
+        }
+        ++n;
+    }
+    *count = n;
+    return RCMPS_OK;
 }
 
 int rcmps_set_kinetic_options(rcmps_context* ctx, const rcmps_kinetic_options* o) {

@@ -18,6 +18,9 @@ struct GemmArgs {
 };
 
 __global__ void rc_zgemm_batched(GemmArgs g);
+template <int WPB>
+__global__ void rc_zgemm_batched_w(GemmArgs g);
+__global__ void rc_zgemm_batched_lds(GemmArgs g);
 __global__ void rc_zgemm_reduce_partial(GemmArgs g, double* partial, int chunk);
 __global__ void rc_sum_partials(const double* partial, int nchunks, int n, double scale, double beta, double* C);
 __global__ void rc_lincomb(const double* X, long long stride, const double* coef, int ncoef, int n, double beta,

@@ -65,6 +65,128 @@ __global__ void __launch_bounds__(256) rc_zgemm_batched(GemmArgs g) {
     C[r * g.Dp + c] = out;
 }
 
+// The same product with one 16x16 output tile per wave and WPB waves per workgroup (grid
+// (ceil(NB*NB / WPB), nbatch)): a wave runs all k-steps itself, keeping the four partial tiles of
+// the k-steps ks = w (mod 4) that rc_zgemm_batched's waves w = 0..3 accumulate, and sums them in
+// the same order, so every variant produces the same bits.  No LDS and no barrier.
+template <int WPB>
+__global__ void __launch_bounds__(64 * WPB) rc_zgemm_batched_w(GemmArgs g) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nb = g.Dp / 16, tile = blockIdx.x * WPB + wave, batch = blockIdx.y;
+    if (tile >= nb * nb) return;
+    const int bi = tile / nb, bj = tile % nb;
+    const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
+    const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
+    cplx* C = reinterpret_cast<cplx*>(g.C) + (long long)batch * g.strideC;
+    cacc acc[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) cacc_zero(acc[w]);
+    const int ks_total = g.Dp / 4;   // a multiple of 4
+    for (int ks = 0; ks < ks_total; ks += 4) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int kk = 4 * (ks + w) + (lane >> 4);
+            cmfma(acc[w], load_op(A, g.Dp, g.opA, 16 * bi + (lane & 15), kk),
+                  load_op(B, g.Dp, g.opB, kk, 16 * bj + (lane & 15)));
+        }
+    }
+    const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * (g.alpha_scale != 0.0 ? g.alpha_scale : 1.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const cplx v = cadd(cadd(cadd(cacc_get(acc[0], q), cacc_get(acc[1], q)), cacc_get(acc[2], q)),
+                            cacc_get(acc[3], q));
+        const int r = 16 * bi + 4 * q + (lane >> 4), c = 16 * bj + (lane & 15);
+        cplx out = cscale(al, v);
+        if (g.beta != 0.0) out = cadd(out, cscale(g.beta, C[r * g.Dp + c]));
+        if (g.addD) {
+            const cplx* Dm = reinterpret_cast<const cplx*>(g.addD) + (long long)batch * g.strideD;
+            const double dl = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
+            out = cadd(out, cscale(dl * g.dscale, Dm[r * g.Dp + c]));
+        }
+        if (g.add_identity != 0.0 && r == c && r < g.D) {
+            const double il = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
+            out.re += il * g.add_identity;
+        }
+        C[r * g.Dp + c] = out;
+    }
+}
+template __global__ void rc_zgemm_batched_w<1>(GemmArgs g);
+template __global__ void rc_zgemm_batched_w<4>(GemmArgs g);
+
+// A 32x32 output block per workgroup of four waves, one 16x16 quadrant per wave (grid
+// (ceil(NB/2)^2, nbatch), dynamic LDS 2 * 32 * (Dp + 1) complex): the block's 32-row panel of op(A)
+// and 32-column panel of op(B) are staged in LDS once with coalesced loads and shared by the
+// waves (each operand element is read once per block instead of once per 16x16 tile).  The k-step
+// classes and their order are those of rc_zgemm_batched, so the bits are the same.
+__global__ void __launch_bounds__(256) rc_zgemm_batched_lds(GemmArgs g) {
+    extern __shared__ cplx panel[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int Dp = g.Dp, nb = Dp / 16, nb2 = (nb + 1) / 2, ld = Dp + 1;
+    const int bI = blockIdx.x / nb2, bJ = blockIdx.x % nb2, batch = blockIdx.y;
+    const int rows = min(32, Dp - 32 * bI), cols = min(32, Dp - 32 * bJ);   // 16 or 32
+    const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
+    const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
+    cplx* C = reinterpret_cast<cplx*>(g.C) + (long long)batch * g.strideC;
+    cplx* Ap = panel;             // Ap[r * ld + k] = op(A)(32 bI + r, k)
+    cplx* Bp = panel + 32 * ld;   // Bp[c * ld + k] = op(B)(k, 32 bJ + c)
+    // op 0 reads A row-major along k; op 1 (conjugate transpose) reads A along the panel's rows
+    for (int i = tid; i < rows * Dp; i += 256) {
+        if (!g.opA) {
+            const int r = i / Dp, k = i % Dp;
+            Ap[r * ld + k] = A[(32 * bI + r) * Dp + k];
+        } else {
+            const int k = i / rows, r = i % rows;
+            const cplx z = A[k * Dp + 32 * bI + r];
+            Ap[r * ld + k] = cmake(z.re, -z.im);
+        }
+    }
+    for (int i = tid; i < cols * Dp; i += 256) {
+        if (!g.opB) {
+            const int k = i / cols, c = i % cols;
+            Bp[c * ld + k] = B[k * Dp + 32 * bJ + c];
+        } else {
+            const int c = i / Dp, k = i % Dp;
+            const cplx z = B[(32 * bJ + c) * Dp + k];
+            Bp[c * ld + k] = cmake(z.re, -z.im);
+        }
+    }
+    __syncthreads();
+    const int qi = wave >> 1, qj = wave & 1;
+    if (16 * qi >= rows || 16 * qj >= cols) return;
+    cacc acc[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) cacc_zero(acc[w]);
+    const int ks_total = Dp / 4;
+    const cplx* Ar = Ap + (16 * qi + (lane & 15)) * ld;
+    const cplx* Bc = Bp + (16 * qj + (lane & 15)) * ld;
+    for (int ks = 0; ks < ks_total; ks += 4) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int kk = 4 * (ks + w) + (lane >> 4);
+            cmfma(acc[w], Ar[kk], Bc[kk]);
+        }
+    }
+    const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * (g.alpha_scale != 0.0 ? g.alpha_scale : 1.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const cplx v = cadd(cadd(cadd(cacc_get(acc[0], q), cacc_get(acc[1], q)), cacc_get(acc[2], q)),
+                            cacc_get(acc[3], q));
+        const int r = 32 * bI + 16 * qi + 4 * q + (lane >> 4), c = 32 * bJ + 16 * qj + (lane & 15);
+        cplx out = cscale(al, v);
+        if (g.beta != 0.0) out = cadd(out, cscale(g.beta, C[r * Dp + c]));
+        if (g.addD) {
+            const cplx* Dm = reinterpret_cast<const cplx*>(g.addD) + (long long)batch * g.strideD;
+            const double dl = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
+            out = cadd(out, cscale(dl * g.dscale, Dm[r * Dp + c]));
+        }
+        if (g.add_identity != 0.0 && r == c && r < g.D) {
+            const double il = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
+            out.re += il * g.add_identity;
+        }
+        C[r * Dp + c] = out;
+    }
+}
+
 // partial[chunk] = sum_{b in chunk} alpha_b op(A_b) op(B_b) ; grid (NB*NB, nchunks)
 __global__ void __launch_bounds__(256) rc_zgemm_reduce_partial(GemmArgs g, double* partial, int chunk) {
     __shared__ cplx red[1024];

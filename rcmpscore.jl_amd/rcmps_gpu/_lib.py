@@ -99,6 +99,10 @@ SIGNATURES = {
                                            ctypes.POINTER(Integrator), ctypes.c_double, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_P), ctypes.POINTER(Stats)]),
     "rcmps_set_kinetic_options": (ctypes.c_int, [_P, ctypes.POINTER(KineticOptions)]),
+    "rcmps_gemm_variant": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rcmps_gemm_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int)]),
     "rcmps_kinetic": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _P, _P, _P,
                                      ctypes.POINTER(Stats)]),
     "rcmps_fg_phi4_rk": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(Integrator), ctypes.c_int,
@@ -200,3 +204,31 @@ def check(status: int, what: str, stats: Stats | None = None):
     if status == 13:
         raise NotImplementedError(f"{what}: configuration not supported by the built kernels")
     raise RuntimeError(f"{what}: {STATUS.get(status, status)}")
+
+
+GEMM_VARIANTS = ("four waves per tile, k split", "four tiles per workgroup", "one tile per one-wave workgroup",
+                 "32x32 block from LDS panels")
+
+
+def gemm_variant(variant: int | None = None, device: int = 0):
+    """Fix the batched-product tile variant of a device's context (0..3), or hand the choice back
+    to the autotuner (None).  Every variant produces the same bits."""
+    v = -1 if variant is None else int(variant)
+    check(load().rcmps_gemm_variant(context(device), v), "gemm variant")
+
+
+def gemm_tuning(device: int = 0):
+    """The autotuner's table: a list of dicts (padded_size, batch_class, variant, us per variant)."""
+    lib = load()
+    ctx = context(device)
+    count = ctypes.c_int()
+    check(lib.rcmps_gemm_tuning(ctx, 0, None, None, None, None, ctypes.byref(count)), "gemm tuning")
+    n = count.value
+    if n == 0:
+        return []
+    dp, cls, var = (ctypes.c_int * n)(), (ctypes.c_int * n)(), (ctypes.c_int * n)()
+    nv = len(GEMM_VARIANTS)
+    us = (ctypes.c_double * (nv * n))()
+    check(lib.rcmps_gemm_tuning(ctx, n, dp, cls, var, us, ctypes.byref(count)), "gemm tuning")
+    return [dict(padded_size=dp[i], batch_class=cls[i], variant=var[i], us=[us[nv * i + v] for v in range(nv)])
+            for i in range(min(n, count.value))]
