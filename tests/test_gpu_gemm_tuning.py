@@ -72,7 +72,8 @@ def test_variants_agree_bit_for_bit(rg):
              f"(us: {', '.join(f'{u:.1f}' for u in row['us'])})")
         assert row["variant"] in (0, 1, 2, 3)
         assert all(u > 0 for u in row["us"])
-        assert row["us"][row["variant"]] == min(row["us"])
+        assert row["variant"] == 0 or row["us"][row["variant"]] < 0.9 * row["us"][0]
+        assert row["variant"] == 0 or row["us"][row["variant"]] == min(row["us"])
     assert {row["padded_size"] for row in table} >= {16, 48}
 
 
