@@ -10,9 +10,11 @@
 #include <complex>
 #include <cstring>
 #include <functional>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rcmps.h"
@@ -192,6 +194,9 @@ struct rcmps_context {
     int gemm_force = -1;
     std::map<std::pair<int, int>, GemmTune> gemm_tuned;
     DevBuf tune_buf;
+    // contexts (own stream and workspaces) on which the per-state pullback, boundary and metric
+    // solves of a batch run concurrently, one host thread each
+    std::vector<rcmps_context*> workers;
 };
 
 struct rcmps_state {
@@ -665,6 +670,8 @@ int rcmps_context_create(int device, rcmps_context** out) {
 
 void rcmps_context_destroy(rcmps_context* ctx) {
     if (!ctx) return;
+    for (auto* w : ctx->workers) rcmps_context_destroy(w);
+    ctx->workers.clear();
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* all[] = {&ctx->traj_params, &ctx->traj_tasks, &ctx->traj_sys, &ctx->traj_sched,
@@ -1728,6 +1735,60 @@ bool valid_integrator(const rcmps_integrator* ip) {
 
 // A batch of evaluations: grouped by padded size, one persistent launch per group, then the
 // pullback of each evaluation.  Every request gets its stats; the first failure is returned.
+// The per-state pullback, boundary and metric solves of one class on up to kFinishWorkers worker
+// contexts, each driven by its own host thread and stream (the states' trajectories are complete
+// and the main stream idle).  Each state is finished by one worker with the same kernels and tile
+// choices as on the main context (every tile variant gives the same bits), so the results equal
+// the sequential ones bit for bit.  Returns false (nothing done) for a single state or when a state
+// appears twice (its cached metric inverse would be shared between threads).
+constexpr int kFinishWorkers = 4;
+
+bool finish_concurrently(rcmps_context* ctx, const rcmps_integrator* ip, double metric_reg, int flags,
+                         const std::vector<int>& idx, std::vector<EvalReq>& reqs) {
+    if (idx.size() < 2) return false;
+    std::vector<const rcmps_state*> seen;
+    for (int i : idx) seen.push_back(reqs[i].st);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) return false;
+    const int nw = std::min<int>(kFinishWorkers, (int)idx.size());
+    while ((int)ctx->workers.size() < nw) {
+        rcmps_context* w = nullptr;
+        const int rc = rcmps_context_create(ctx->device, &w);
+        if (rc != RCMPS_OK) throw Fail{rc};
+        w->gemm_force = ctx->gemm_force;
+        ctx->workers.push_back(w);
+    }
+    for (int w = 0; w < nw; ++w) {
+        ctx->workers[w]->gemm_force = ctx->gemm_force;
+        ensure_quadrature(ctx->workers[w], ip);
+    }
+    std::atomic<int> next{0};
+    auto work = [&](rcmps_context* wc) {
+        (void)hipSetDevice(wc->device);
+        for (;;) {
+            const int j = next.fetch_add(1);
+            if (j >= (int)idx.size()) break;
+            EvalReq& r = reqs[idx[j]];
+            if (r.S->status != RCMPS_OK || !r.adjoint()) continue;
+            int code = RCMPS_OK;
+            try {
+                code = guard([&] {
+                    finish_state(wc, r, *ctx->tb[j], ip, metric_reg, flags);
+                    ck(hipStreamSynchronize(wc->stream));
+                });
+            } catch (...) {   // nothing may leave a worker thread
+                code = RCMPS_ERR_DEVICE;
+            }
+            if (code != RCMPS_OK) r.S->status = code;
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int w = 1; w < nw; ++w) threads.emplace_back(work, ctx->workers[w]);
+    work(ctx->workers[0]);
+    for (auto& t : threads) t.join();
+    return true;
+}
+
 int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcmps_integrator* ip,
                        double metric_reg, int flags) {
     if (!ctx || !valid_integrator(ip) || !(metric_reg >= 0)) return RCMPS_ERR_ARGUMENT;
@@ -1749,6 +1810,7 @@ int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcm
                 if (reqs[i].st->Dp == Dp) idx.push_back((int)i);
             if (idx.empty()) continue;
             run_class(ctx, Dp, (flags & RCMPS_FLAG_SPLIT) != 0, krylov, ip, idx, reqs);
+            if (!krylov && finish_concurrently(ctx, ip, metric_reg, flags, idx, reqs)) continue;
             for (size_t j = 0; j < idx.size(); ++j) {
                 EvalReq& r = reqs[idx[j]];
                 if (r.S->status != RCMPS_OK) continue;
