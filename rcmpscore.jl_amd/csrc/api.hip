@@ -1187,11 +1187,11 @@ void finish_state(rcmps_context* ctx, EvalReq& r, TrajBufs& tb, const rcmps_inte
         ck(hipMemsetAsync(T1, 0, 3 * mb, ctx->stream));   // T1, T2, U
         if (term < 2) {
             // k = 1 (F1 = rho):  T1 += rho R^+ (sum w A1), T2 += rho (sum w A1)
-            hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Aptr(0), -astride,
+            hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, Aptr(0), -astride,
                                ctx->nodes_w.d(), nn, dv.n(), 0.0, X1);
-            hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Aptr(1), -astride,
+            hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, Aptr(1), -astride,
                                ctx->nodes_j.d(), nn, dv.n(), 0.0, X2);           // sum j A2
-            hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, Fptr(0), fstride,
+            hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, Fptr(0), fstride,
                                ctx->nodes_j.d(), nn, dv.n(), 0.0, tmp2);         // sum j F2
             ck(hipGetLastError());
             dv.gemm(0, st->Rh.d(), 0, X1, tmp);
@@ -1564,7 +1564,7 @@ void kinetic_components(rcmps_context* ctx, const rcmps_state* st, Dev& dv, int 
     // forward solves on rho op^+
     dv.gemm(0, st->rho.d(), 0, opd, rhs);
     kinetic_solutions(ctx, st, dv, kw.basis[2 * commutator], rhs, false, z, kw.X.d());
-    hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, kw.X.d(), node, kw.wd.d(), nz, dv.n(), 0.0, u);
+    hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, kw.X.d(), node, kw.wd.d(), nz, dv.n(), 0.0, u);
     ck(hipGetLastError());
     dv.trace_prod(op, u, ctx->small.d());
     value = dv.fetch2(ctx->small.d()).real();   // sum_i w_i Re tr(op x_i)
@@ -1590,7 +1590,7 @@ void kinetic_components(rcmps_context* ctx, const rcmps_state* st, Dev& dv, int 
     GemmArgs s4{};   // S4 = sum wz a_i x_i
     s4.opA = 1; s4.A = kw.A.d(); s4.strideA = node; s4.B = kw.X.d(); s4.strideB = node; s4.alpha = kw.wzd.d(); s4.alpha_stride = 1;
     dv.reduce(s4, nz, S4, 1.0, 0.0);
-    hipLaunchKernelGGL(rc_lincomb, dv.eb(), dim3(256), 0, ctx->stream, kw.A.d(), node, kw.wd.d(), nz, dv.n(), 0.0, Sa);
+    hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, kw.A.d(), node, kw.wd.d(), nz, dv.n(), 0.0, Sa);
     ck(hipGetLastError());
     dv.herm(1, 1.0, Sa, a);                                  // a = sum w a_i
     dv.gemm(0, S2, 0, st->Rh.d(), x2);                       // m = S1 - S2 R^+

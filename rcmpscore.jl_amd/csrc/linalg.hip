@@ -229,13 +229,27 @@ __global__ void rc_sum_partials(const double* partial, int nchunks, int n, doubl
 }
 
 // out = sum_k coef[k] * X_k   (X_k = X + k*stride), n complex elements; coef real
+// Sixteen terms' loads are issued together before they are added in order k = 0, 1, ..., so the
+// node sums of the pullback overlap their memory latency and keep the sequential sum's bits.
 __global__ void rc_lincomb(const double* X, long long stride, const double* coef, int ncoef, int n,
                            double beta, double* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const cplx* x = reinterpret_cast<const cplx*>(X);
     cplx s = cmake(0, 0);
-    for (int k = 0; k < ncoef; ++k) s = cadd(s, cscale(coef[k], x[(long long)k * stride + i]));
+    int k = 0;
+    for (; k + 16 <= ncoef; k += 16) {
+        cplx v[16];
+        double c[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            v[q] = x[(long long)(k + q) * stride + i];
+            c[q] = coef[k + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s = cadd(s, cscale(c[q], v[q]));
+    }
+    for (; k < ncoef; ++k) s = cadd(s, cscale(coef[k], x[(long long)k * stride + i]));
     cplx* o = reinterpret_cast<cplx*>(out);
     if (beta != 0.0) s = cadd(s, cscale(beta, o[i]));
     o[i] = s;
