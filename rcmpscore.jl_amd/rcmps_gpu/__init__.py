@@ -15,6 +15,9 @@ from .io import StateRecord, rcmps_load, rcmps_store, read_portable, write_porta
 from .observables import (boundary_environment, fg_phi4, fg_phi4_batch, fg_phi4_raw, fg_phi4_rk,
                           trajectory_samples)
 from .linesearch import BackwardLineSearch, HagerZhangLineSearch, SafeguardedLineSearch
+from . import controller
+from .controller import (OptimizationControls, RobustnessControls, gpu_objective, gpu_stationary,
+                         load_resume_state, save_resume_state)
 from .optimization import metric, optimize_state, precondition, retraction
 from .state import (IntegratorParameters, ManifoldParameters, ModelParameters, RandomState, RandomStateZ2,
                     State, random_state_arrays, random_state_z2_arrays)
@@ -31,5 +34,6 @@ __all__ = [
     "action_dual_poisson", "davidson", "dispersion_relation", "excitation_eigsolve", "excitation_hamiltonian_action",
     "excitation_parity", "excitation_poisson", "field_seed", "iterative_dispersion", "kinetic_preconditioner",
     "relativistic_diagnostics", "excitation_element", "tangent_metric", "excitation_matrix_element",
-    "gemm_tuning", "gemm_variant",
+    "gemm_tuning", "gemm_variant", "controller", "OptimizationControls", "RobustnessControls", "gpu_objective",
+    "gpu_stationary", "load_resume_state", "save_resume_state",
 ]

@@ -131,3 +131,36 @@ def test_lbfgs_steps_match_the_oracle(rg, monkeypatch):
     assert a["evaluations"] == b["evaluations"] and len(ca) == len(cb)
     assert np.abs(np.array(ca) - np.array(cb)).max() < 1e-10
     assert a["energy"] < ca[0]
+
+
+def test_controller_resume_on_the_gpu(rg, tmp_path):
+    """The resumable controller with fg_phi4 and the stationary state on the GPU: energies fall at
+    every accepted step, and four iterations equal two, a saved snapshot, and two more, bit for bit."""
+    note("case: controller D=8 g=2 seed=5, 4 iterations vs 2 + resume 2")
+    from rcmps_gpu.controller import OptimizationControls, RobustnessControls, optimize
+    r, k = rg.random_state_arrays(8, scaling=0.2, seed=5)
+    s = rg.State(r, k, steady_tol=1e-13)
+    ip = rg.IntegratorParameters(n_quad=64, max_point=20.0, atol=1e-11, rtol=1e-10)
+    s.integrator_param = ip
+    e0, g0 = rg.fg_phi4(s, 2.0)
+    objective, stationary = rg.gpu_objective(2.0, integrator=ip), rg.gpu_stationary()
+    robust = RobustnessControls(line_search="wolfe", wolfe_energy_atol=1e-10)
+
+    def events(n, resume=None):
+        out = list(optimize(s.R, s.K, s.rho, e0, g0, objective, OptimizationControls(max_iterations=n, history=5),
+                            resume_state=resume, robustness=robust, stationary=stationary))
+        assert out[-1]["type"] == "stopped"
+        return out
+
+    full = events(4)
+    energies = [e0] + [ev["energy"] for ev in full if ev["type"] == "accepted"]
+    note(f"  energies {energies}")
+    assert len(energies) == 5 and energies[-1] < energies[0]
+    half = events(2)
+    path = rg.save_resume_state(tmp_path / "state.npz", half[-1]["state"])
+    rest = events(4, rg.load_resume_state(path))
+    a, b = full[-1]["state"], rest[-1]["state"]
+    for key in ("R", "K", "rho", "gradient"):
+        assert np.array_equal(a[key], b[key])
+    assert a["energy"] == b["energy"] and a["evaluations"] == b["evaluations"]
+    s.close()
