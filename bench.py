@@ -198,8 +198,8 @@ def excitation_main(args):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 5; 1 with --excitation)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; 0 with --excitation)")
     ap.add_argument("--D", type=int, default=64)
     ap.add_argument("--n-quad", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -214,6 +214,14 @@ def main(argv=None):
     ap.add_argument("--exc-D", type=int, default=32)
     ap.add_argument("--exc-chunk", type=int, default=128, help="tangents per coarse kinetic action call")
     args = ap.parse_args(argv)
+    # the dispersion solve is minutes long: one timed step and no warmup unless asked for
+    if args.steps is None:
+        args.steps = 1 if args.excitation else 5
+    if args.warmup is None:
+        args.warmup = 0 if args.excitation else 2
+    if args.steps < 1 or args.warmup < 0:
+        print("bench.py: --steps must be at least 1 and --warmup at least 0", file=sys.stderr)
+        return 2
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent states shard across GPUs with no exchange)", file=sys.stderr)
