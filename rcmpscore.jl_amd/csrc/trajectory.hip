@@ -313,8 +313,12 @@ __device__ __forceinline__ int max_exp4(const cplx* v) {
 // The lane holding k = 0 writes it to sg[row or column].
 __device__ __forceinline__ int line_scale(const cplx* v, int* sg) {
     int e = max_exp4(v);
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+    // 16-lane maximum by DPP moves within each row of 16 lanes: quad_perm [1,0,3,2] and [2,3,0,1]
+    // give the quad's maximum, row_half_mirror joins quads 0-1 and 2-3, row_mirror the two halves
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0xB1, 0xF, 0xF, false));
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0x4E, 0xF, 0xF, false));
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0x141, 0xF, 0xF, false));
+    e = max(e, __builtin_amdgcn_update_dpp(0, e, 0x140, 0xF, 0xF, false));
     const int S = rc_split_scale_e(e);
     if ((threadIdx.x & 15) == 0) sg[threadIdx.x >> 4] = S;
     return S;
