@@ -109,15 +109,50 @@ __device__ __forceinline__ cplx reduce_tile(const cacc& acc, cplx* red, int wave
 }
 
 // Deterministic block sum (same inputs give the same bits in every workgroup).
+// a DPP move of a double within each row of 16 lanes (CTRL a compile-time DPP control)
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double read_lane(double x, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                            __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+// Wave sum in a fixed order: pairs, quads (quad_perm), halves (row_half_mirror) and rows
+// (row_mirror) by DPP moves, every lane of a row holding the same row sum (addition commutes),
+// then the four row sums in order.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
+    return ((read_lane(v, 0) + read_lane(v, 16)) + read_lane(v, 32)) + read_lane(v, 48);
+}
+
 __device__ __forceinline__ double block_sum(double v, double* scratch) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    v = wave_sum(v);
     if (lane == 0) scratch[wave] = v;
     __syncthreads();
     double s = ((scratch[0] + scratch[1]) + scratch[2]) + scratch[3];
     __syncthreads();
     return s;
+}
+// two block sums with one exchange
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* scratch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (lane == 0) {
+        scratch[wave] = a;
+        scratch[4 + wave] = b;
+    }
+    __syncthreads();
+    a = ((scratch[0] + scratch[1]) + scratch[2]) + scratch[3];
+    b = ((scratch[4] + scratch[5]) + scratch[6]) + scratch[7];
+    __syncthreads();
 }
 
 // Load a row block (rows 16b.., all Dp columns) of a global Dp x Dp matrix into M[16][RS].
@@ -725,8 +760,7 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
                 e5p = cabs2(cmake(acc5.re / sc, acc5.im / sc));
                 e3p = cabs2(cmake(acc3.re / sc, acc3.im / sc));
             }
-            e5p = block_sum(e5p, scratch);
-            e3p = block_sum(e3p, scratch);
+            block_sum2(e5p, e3p, scratch);
             // record for the all-gather: e5, e3, trace partials of k_0..k_11
             const uint32_t par = (uint32_t)(attempts & 1);
             if (tid < RC_REC) {
