@@ -154,3 +154,26 @@ def test_golden_d32_checkpoint(rg, precision):
     assert abs(e - (-0.30190930265518967)) < 3e-10
     assert np.abs(g - s.grad).max() < 1e-6
     s.close()
+
+
+def test_limits_and_invalid_inputs(rg):
+    """Edge cases: a bond dimension above the built kernels (D = 65) and a window beyond the source
+    table are refused as unsupported (no computation), a non-Hermitian K and an empty batch as
+    invalid arguments, and D = 64, the largest built size, evaluates."""
+    note("case: limits D=65, max_point=64.5, non-Hermitian K, empty batch")
+    r, k = rg.random_state_arrays(65, scaling=0.2, seed=1)
+    s = rg.State(r, k)
+    with pytest.raises(NotImplementedError):
+        rg.fg_phi4(s, 2.0)
+    s.close()
+    r, k = rg.random_state_arrays(4, scaling=0.3, seed=2)
+    s = rg.State(r, k)
+    s.integrator_param.max_point = 64.5   # sqrt beyond the source table's range
+    with pytest.raises(NotImplementedError):
+        rg.fg_phi4(s, 2.0)
+    s.close()
+    with pytest.raises(ValueError):
+        rg.State(r, k + 0.1j * np.eye(4))
+    with pytest.raises(ValueError):
+        rg.fg_phi4_batch([], [])
+    assert rg._lib.load().rcmps_max_bond_dimension() == 64
