@@ -174,6 +174,10 @@ struct rcmps_context {
     DevBuf nodes_w, nodes_j, tmpM, partial, scratch, small;
     // krylov workspaces
     DevBuf kry_V, kry_w, kry_t1, kry_t2, kry_x, kry_b, kry_z, kry_h, kry_tmp;
+    // GMRES read-back: two pinned slots of one Hessenberg column each, and their completion events
+    double* kry_pinned = nullptr;
+    size_t kry_pinned_doubles = 0;
+    hipEvent_t kry_ev[2] = {nullptr, nullptr};
     int last_D = 0, last_Dp = 0, last_npts = 0;
     bool last_adjoint = false;   // the last evaluation integrated the adjoint systems
     // quadrature cache: nodes (plus endpoint) and the pullback weights on the device
@@ -453,19 +457,32 @@ void lindblad(Dev& dv, const rcmps_state* st, const double* X, double* out, bool
 }
 
 // Restarted GMRES(m) with classical Gram-Schmidt twice; vectors on the device, Hessenberg on the
-// host.  Solves A x = b; returns true when the true residual is <= tol_abs.
+// host.  Solves A x = b; returns true when the true residual is <= tol_abs.  The Arnoldi vectors
+// never depend on the host's Givens rotations, so step k + 1 is queued before the host reads step
+// k's Hessenberg column (two pinned slots): the host's rotations overlap the next step's products.
+// A step queued past convergence only fills a basis vector the solution does not use, so the
+// iterates, the results and the matvec count are those of the unpipelined iteration.
 bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const double* b, double* x,
            double tol_abs, int m, int maxiter, int& matvecs, double& resid) {
     rcmps_context* ctx = dv.ctx;
     const int n = dv.n();
+    const size_t col = (size_t)4 * (m + 1) + 2;   // doubles per read-back: h1, h2 (complex), <w,w>
     ck(ctx->kry_V.ensure((size_t)(m + 1) * n * 16));
     ck(ctx->kry_w.ensure((size_t)n * 16));
-    ck(ctx->kry_h.ensure((size_t)(4 * (m + 1) + 2) * 8 + 64));
+    ck(ctx->kry_h.ensure(col * 8 + 64));
+    if (ctx->kry_pinned_doubles < 2 * col) {
+        ck(hipStreamSynchronize(ctx->stream));
+        if (ctx->kry_pinned) ck(hipHostFree(ctx->kry_pinned));
+        ctx->kry_pinned = nullptr;
+        ctx->kry_pinned_doubles = 0;
+        ck(hipHostMalloc(reinterpret_cast<void**>(&ctx->kry_pinned), 2 * col * 8));
+        ctx->kry_pinned_doubles = 2 * col;
+    }
     double* V = ctx->kry_V.d();
     double* w = ctx->kry_w.d();
     double* hdev = ctx->kry_h.d();
     ck(hipMemsetAsync(x, 0, (size_t)n * 16, ctx->stream));
-    std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yk(m), hv(2 * (m + 1) + 1);
+    std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yk(m);
     std::vector<double> cs(m);
     matvecs = 0;
     bool converged = false;
@@ -474,6 +491,24 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
         ++matvecs;
         dv.axpby(cd(1, 0), b, cd(-1, 0), w, w);
         return dv.norm(w);
+    };
+    // Arnoldi step k: w = A v_k, classical Gram-Schmidt twice on the device (h1, h2: k + 1 complex
+    // each, then <w,w>), v_{k+1} = w / |w|, and the column copied to pinned slot k & 1
+    auto queue_step = [&](int k) {
+        A(V + (size_t)k * n * 2, w);
+        double* hd1 = hdev;
+        double* hd2 = hdev + 2 * (m + 1);
+        double* hnn = hdev + 4 * (m + 1);
+        hipLaunchKernelGGL(rc_dots, dim3(k + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd1);
+        hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, k + 1, hd1, n, w);
+        hipLaunchKernelGGL(rc_dots, dim3(k + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd2);
+        hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, k + 1, hd2, n, w);
+        hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hnn);
+        hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(k + 1) * n * 2);
+        ck(hipGetLastError());
+        ck(hipMemcpyAsync(ctx->kry_pinned + (size_t)(k & 1) * col, hdev, col * 8, hipMemcpyDeviceToHost,
+                          ctx->stream));
+        ck(hipEventRecord(ctx->kry_ev[k & 1], ctx->stream));
     };
     double beta = residual_into_w();
     resid = beta;
@@ -487,22 +522,12 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
         std::fill(g.begin(), g.end(), cd(0, 0));
         g[0] = beta;
         int k = 0;
+        queue_step(0);
         for (; k < m; ++k) {
-            A(V + (size_t)k * n * 2, w);
+            if (k + 1 < m) queue_step(k + 1);
+            ck(hipEventSynchronize(ctx->kry_ev[k & 1]));
             ++matvecs;
-            // classical Gram-Schmidt twice on the device: h1, h2 (k+1 complex each), then <w,w>
-            double* hd1 = hdev;
-            double* hd2 = hdev + 2 * (m + 1);
-            double* hnn = hdev + 4 * (m + 1);
-            hipLaunchKernelGGL(rc_dots, dim3(k + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd1);
-            hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, k + 1, hd1, n, w);
-            hipLaunchKernelGGL(rc_dots, dim3(k + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd2);
-            hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, k + 1, hd2, n, w);
-            hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hnn);
-            hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(k + 1) * n * 2);
-            ck(hipGetLastError());
-            ck(hipMemcpyAsync(hv.data(), hdev, (size_t)(4 * (m + 1) + 2) * 8, hipMemcpyDeviceToHost, ctx->stream));
-            ck(hipStreamSynchronize(ctx->stream));
+            const cd* hv = reinterpret_cast<const cd*>(ctx->kry_pinned + (size_t)(k & 1) * col);
             for (int i = 0; i <= k; ++i)
                 H[(size_t)i * m + k] += hv[i] + hv[(m + 1) + i];
             double hn = std::sqrt(std::max(0.0, hv[2 * (m + 1)].real()));
@@ -651,6 +676,7 @@ int rcmps_context_create(int device, rcmps_context** out) {
         ck(hipSetDevice(device));
         ck(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         for (auto& e : ctx->ev) ck(hipEventCreate(&e));
+        for (auto& e : ctx->kry_ev) ck(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         ck(ctx->small.ensure(4096));
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
@@ -683,6 +709,9 @@ void rcmps_context_destroy(rcmps_context* ctx) {
     for (auto& b : ctx->tb) b->release();
     ctx->kin.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+    for (auto& e : ctx->kry_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->kry_pinned) (void)hipHostFree(ctx->kry_pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
