@@ -203,6 +203,9 @@ def main(argv=None):
     ap.add_argument("--D", type=int, default=64)
     ap.add_argument("--n-quad", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gemm-variant", type=int, default=None, metavar="V",
+                    help="force batched-product tile variant V (0-3) instead of the autotuner's choice "
+                         "(a profiler's per-dispatch overhead distorts the tuner's timings)")
     ap.add_argument("--cpu-attempts", type=int, default=10)
     ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of this many MiB (a known-size kernel that "
@@ -242,6 +245,8 @@ def main(argv=None):
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.gemm_variant is not None:
+        rcmps_gpu.gemm_variant(args.gemm_variant)
     wl = workload(D=args.D, n_quad=args.n_quad)
     r, k = rcmps_gpu.random_state_arrays(wl["D"], scaling=wl["scaling"], seed=wl["seed"])
     t0 = time.perf_counter()
@@ -306,6 +311,10 @@ def main(argv=None):
         "attempts": stats["attempts"], "dense_steps": stats["dense_steps"],
         "trajectory_ms": split["traj_s"] * 1e3, "pullback_ms": stats["pullback_ms"],
         "boundary_ms": stats["boundary_ms"], "state_prep_s": prep_s,
+        # the batched-product tile variant per (padded size, batch class): picked by the autotuner
+        # on first use, or forced with --gemm-variant
+        "gemm_variants": {f"Dp{t['padded_size']}/c{t['batch_class']}": t["variant"]
+                          for t in rcmps_gpu.gemm_tuning()},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s.R, s.Q, s.rho, wl, stats, args.cpu_attempts)

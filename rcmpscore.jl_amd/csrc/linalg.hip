@@ -14,8 +14,43 @@ __device__ __forceinline__ cplx load_op(const cplx* M, int Dp, int op, int r, in
     return cmake(z.re, -z.im);
 }
 
+// A wave's share of one tile product: k-steps ks = wave, wave + 4, ... (at most four for Dp <= 64).
+// All fragments are loaded before the first MFMA, so the wave waits for one round trip instead of
+// one per k-step; the MFMAs then run in increasing ks, the order of the plain loop.
+struct TileFrags {
+    cplx a[4], b[4];
+    double alpha;
+};
+
+__device__ __forceinline__ void tile_load(TileFrags& f, const cplx* A, int opA, const cplx* B, int opB, int Dp,
+                                          int bi, int bj, int wave, int lane) {
+    const int ks_total = Dp / 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int ks = wave + 4 * j;
+        if (ks < ks_total) {
+            const int kk = 4 * ks + (lane >> 4);
+            f.a[j] = load_op(A, Dp, opA, 16 * bi + (lane & 15), kk);
+            f.b[j] = load_op(B, Dp, opB, kk, 16 * bj + (lane & 15));
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_fma(cacc& acc, const TileFrags& f, int Dp, int wave) {
+    const int ks_total = Dp / 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (wave + 4 * j < ks_total) cmfma(acc, f.a[j], f.b[j]);
+}
+
 __device__ __forceinline__ void tile_mma(cacc& acc, const cplx* A, int opA, const cplx* B, int opB,
                                          int Dp, int bi, int bj, int wave, int lane) {
+    if (Dp <= 64) {
+        TileFrags f;
+        tile_load(f, A, opA, B, opB, Dp, bi, bj, wave, lane);
+        tile_fma(acc, f, Dp, wave);
+        return;
+    }
     const int ks_total = Dp / 4;
     for (int ks = wave; ks < ks_total; ks += 4) {
         const int kk = 4 * ks + (lane >> 4);
@@ -188,26 +223,55 @@ __global__ void __launch_bounds__(256) rc_zgemm_batched_lds(GemmArgs g) {
 }
 
 // partial[chunk] = sum_{b in chunk} alpha_b op(A_b) op(B_b) ; grid (NB*NB, nchunks)
+// The next batch's fragments and weight are loaded while the current batch's MFMAs run; the
+// batches are still accumulated one after another in increasing b.
 __global__ void __launch_bounds__(256) rc_zgemm_reduce_partial(GemmArgs g, double* partial, int chunk) {
     __shared__ cplx red[1024];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nb = g.Dp / 16, bi = blockIdx.x / nb, bj = blockIdx.x % nb;
     const int b0 = blockIdx.y * chunk, b1 = min(g.nbatch, b0 + chunk);
-    cacc acc;
-    cacc_zero(acc);
-    for (int batch = b0; batch < b1; ++batch) {
+    const double ascale = g.alpha_scale != 0.0 ? g.alpha_scale : 1.0;
+    auto fetch = [&](TileFrags& f, int batch) {
         const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
         const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
-        const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * (g.alpha_scale != 0.0 ? g.alpha_scale : 1.0);
-        cacc part;
-        cacc_zero(part);
-        tile_mma(part, A, g.opA, B, g.opB, g.Dp, bi, bj, wave, lane);
+        f.alpha = g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0;
+        tile_load(f, A, g.opA, B, g.opB, g.Dp, bi, bj, wave, lane);
+    };
+    cacc acc;
+    cacc_zero(acc);
+    if (g.Dp <= 64 && b0 < b1) {
+        TileFrags cur, nxt;
+        fetch(cur, b0);
+        for (int batch = b0; batch < b1; ++batch) {
+            if (batch + 1 < b1) fetch(nxt, batch + 1);
+            const double al = cur.alpha * ascale;
+            cacc part;
+            cacc_zero(part);
+            tile_fma(part, cur, g.Dp, wave);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            acc.rr[q] += al * part.rr[q];
-            acc.ii[q] += al * part.ii[q];
-            acc.ri[q] += al * part.ri[q];
-            acc.ir[q] += al * part.ir[q];
+            for (int q = 0; q < 4; ++q) {
+                acc.rr[q] += al * part.rr[q];
+                acc.ii[q] += al * part.ii[q];
+                acc.ri[q] += al * part.ri[q];
+                acc.ir[q] += al * part.ir[q];
+            }
+            cur = nxt;
+        }
+    } else {
+        for (int batch = b0; batch < b1; ++batch) {
+            const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
+            const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
+            const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * ascale;
+            cacc part;
+            cacc_zero(part);
+            tile_mma(part, A, g.opA, B, g.opB, g.Dp, bi, bj, wave, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc.rr[q] += al * part.rr[q];
+                acc.ii[q] += al * part.ii[q];
+                acc.ri[q] += al * part.ri[q];
+                acc.ir[q] += al * part.ir[q];
+            }
         }
     }
     cplx v = reduce4(acc, red, wave, lane);
