@@ -70,13 +70,33 @@ __device__ __forceinline__ cplx reduce4(const cacc& acc, cplx* red, int wave, in
     return s;
 }
 
+// The (tile, batch) of a workgroup in a grid (tiles, batches).  Workgroups go to the 8 XCDs
+// round-robin in linear order (x fastest); when the batches are a multiple of 8, the tiles of one
+// batch are given to workgroups of one XCD, so its L2 serves the batch's operand panels to all of
+// them instead of every XCD reading them from HBM.  A bijection: each (tile, batch) is computed
+// exactly once, by the same arithmetic, so only the placement changes.
+__device__ __forceinline__ void xcd_tile_batch(int& tile, int& batch) {
+    const int nt = gridDim.x;
+    if ((gridDim.y & 7) == 0) {
+        const int lin = blockIdx.x + nt * blockIdx.y;
+        const int slot = lin >> 3;
+        batch = (lin & 7) + 8 * (slot / nt);
+        tile = slot % nt;
+    } else {
+        tile = blockIdx.x;
+        batch = blockIdx.y;
+    }
+}
+
 }  // namespace
 
 // C_b = alpha_b op(A_b) op(B_b) + beta C_b ; grid (NB*NB, nbatch)
 __global__ void __launch_bounds__(256) rc_zgemm_batched(GemmArgs g) {
     __shared__ cplx red[1024];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nb = g.Dp / 16, bi = blockIdx.x / nb, bj = blockIdx.x % nb, batch = blockIdx.y;
+    int tile, batch;
+    xcd_tile_batch(tile, batch);
+    const int nb = g.Dp / 16, bi = tile / nb, bj = tile % nb;
     const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
     const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
     cplx* C = reinterpret_cast<cplx*>(g.C) + (long long)batch * g.strideC;
@@ -228,8 +248,10 @@ __global__ void __launch_bounds__(256) rc_zgemm_batched_lds(GemmArgs g) {
 __global__ void __launch_bounds__(256) rc_zgemm_reduce_partial(GemmArgs g, double* partial, int chunk) {
     __shared__ cplx red[1024];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nb = g.Dp / 16, bi = blockIdx.x / nb, bj = blockIdx.x % nb;
-    const int b0 = blockIdx.y * chunk, b1 = min(g.nbatch, b0 + chunk);
+    int tile, ci;
+    xcd_tile_batch(tile, ci);
+    const int nb = g.Dp / 16, bi = tile / nb, bj = tile % nb;
+    const int b0 = ci * chunk, b1 = min(g.nbatch, b0 + chunk);
     const double ascale = g.alpha_scale != 0.0 ? g.alpha_scale : 1.0;
     auto fetch = [&](TileFrags& f, int batch) {
         const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
@@ -276,7 +298,7 @@ __global__ void __launch_bounds__(256) rc_zgemm_reduce_partial(GemmArgs g, doubl
     }
     cplx v = reduce4(acc, red, wave, lane);
     const int r = 16 * bi + (tid >> 4), c = 16 * bj + (tid & 15);
-    reinterpret_cast<cplx*>(partial)[(size_t)blockIdx.y * g.Dp * g.Dp + r * g.Dp + c] = v;
+    reinterpret_cast<cplx*>(partial)[(size_t)ci * g.Dp * g.Dp + r * g.Dp + c] = v;
 }
 
 // C = beta C + scale * sum_chunks partial (fixed order)
