@@ -381,7 +381,36 @@ __global__ void rc_sylvester_diag(int Dp, int D, int adjoint, double shift, cons
     z[i] = cmake(-q.re, -q.im);
 }
 
-// Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.
+// Sum of the 256 threads' (re, im) into out[0..1] by the pairwise tree t += t + s, s = 128, 64,
+// ..., 1: the first two levels through LDS, the last six inside wave 0 by shuffles (lane t adds
+// lane t + s's value to its own, the operands and order of the LDS tree).
+__device__ __forceinline__ void tree_sum_256(double re, double im, double* sre, double* sim, double* out) {
+    const int t = threadIdx.x;
+    sre[t] = re;
+    sim[t] = im;
+    __syncthreads();
+    if (t < 128) {
+        sre[t] += sre[t + 128];
+        sim[t] += sim[t + 128];
+    }
+    __syncthreads();
+    if (t < 64) {
+        re = sre[t] + sre[t + 64];
+        im = sim[t] + sim[t + 64];
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) {
+            re += __shfl_down(re, s);
+            im += __shfl_down(im, s);
+        }
+        if (t == 0) {
+            out[0] = re;
+            out[1] = im;
+        }
+    }
+}
+
+// Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.  Thread t
+// adds elements t, t + 256, ... in increasing order, eight elements' loads issued together.
 __global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride, int n, const double* w,
                                                double* out) {
     __shared__ double sre[256], sim[256];
@@ -389,19 +418,22 @@ __global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride
     const cplx* v = reinterpret_cast<const cplx*>(V) + (size_t)k * stride;
     const cplx* ww = reinterpret_cast<const cplx*>(w);
     double re = 0, im = 0;
-    for (int i = t; i < n; i += 256) {
-        cplx a = v[i], b = ww[i];
-        re += a.re * b.re + a.im * b.im;
-        im += a.re * b.im - a.im * b.re;
+    for (int i0 = t; i0 < n; i0 += 8 * 256) {
+        cplx a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 256 * u < n) {
+                a[u] = v[i0 + 256 * u];
+                b[u] = ww[i0 + 256 * u];
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 256 * u < n) {
+                re += a[u].re * b[u].re + a[u].im * b[u].im;
+                im += a[u].re * b[u].im - a[u].im * b[u].re;
+            }
     }
-    sre[t] = re;
-    sim[t] = im;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) { sre[t] += sre[t + s]; sim[t] += sim[t + s]; }
-        __syncthreads();
-    }
-    if (t == 0) { out[2 * k] = sre[0]; out[2 * k + 1] = sim[0]; }
+    tree_sum_256(re, im, sre, sim, out + 2 * k);
 }
 
 // w -= sum_k h_k V_k   (h complex, host-provided array in device memory)
@@ -426,27 +458,34 @@ __global__ void rc_add_combo(const double* V, long long stride, int nk, const do
     reinterpret_cast<cplx*>(x)[i] = s;
 }
 
-// out[0] = trace(A B) for Dp x Dp (single workgroup)
+// out[0] = trace(A B) for Dp x Dp (single workgroup).  Thread t adds the products of entries
+// idx = t, t + 256, ... in increasing order, eight entries' loads issued together.
 __global__ void __launch_bounds__(256) rc_trace_prod(int Dp, const double* A, const double* B, double* out) {
     __shared__ double sre[256], sim[256];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, n = Dp * Dp;
     const cplx* a = reinterpret_cast<const cplx*>(A);
     const cplx* b = reinterpret_cast<const cplx*>(B);
     double re = 0, im = 0;
-    for (int idx = t; idx < Dp * Dp; idx += 256) {
-        const int i = idx / Dp, k = idx % Dp;
-        cplx p = cmul(a[i * Dp + k], b[k * Dp + i]);
-        re += p.re;
-        im += p.im;
+    for (int i0 = t; i0 < n; i0 += 8 * 256) {
+        cplx x[8], y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = i0 + 256 * u;
+            if (idx < n) {
+                const int i = idx / Dp, k = idx % Dp;
+                x[u] = a[i * Dp + k];
+                y[u] = b[k * Dp + i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 256 * u < n) {
+                cplx p = cmul(x[u], y[u]);
+                re += p.re;
+                im += p.im;
+            }
     }
-    sre[t] = re;
-    sim[t] = im;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) { sre[t] += sre[t + s]; sim[t] += sim[t + s]; }
-        __syncthreads();
-    }
-    if (t == 0) { out[0] = sre[0]; out[1] = sim[0]; }
+    tree_sum_256(re, im, sre, sim, out);
 }
 
 // X += (c_re + i c_im) I on the D x D block, c read from device memory at cptr (scaled)
