@@ -307,7 +307,16 @@ __global__ void rc_sum_partials(const double* partial, int nchunks, int n, doubl
     if (i >= n) return;
     const cplx* p = reinterpret_cast<const cplx*>(partial);
     cplx s = cmake(0, 0);
-    for (int k = 0; k < nchunks; ++k) s = cadd(s, p[(size_t)k * n + i]);
+    // eight chunks' loads in flight, added in increasing k
+    for (int k0 = 0; k0 < nchunks; k0 += 8) {
+        cplx x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < nchunks) x[u] = p[(size_t)(k0 + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < nchunks) s = cadd(s, x[u]);
+    }
     cplx* Cc = reinterpret_cast<cplx*>(C);
     cplx out = cscale(scale, s);
     if (beta != 0.0) out = cadd(out, cscale(beta, Cc[i]));
@@ -443,7 +452,19 @@ __global__ void rc_subtract_combo(const double* V, long long stride, int nk, con
     const cplx* v = reinterpret_cast<const cplx*>(V);
     const cplx* hh = reinterpret_cast<const cplx*>(h);
     cplx s = reinterpret_cast<cplx*>(w)[i];
-    for (int k = 0; k < nk; ++k) s = csub(s, cmul(hh[k], v[(size_t)k * stride + i]));
+    // eight vectors' loads in flight, subtracted in increasing k
+    for (int k0 = 0; k0 < nk; k0 += 8) {
+        cplx x[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < nk) {
+                x[u] = v[(size_t)(k0 + u) * stride + i];
+                c[u] = hh[k0 + u];
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < nk) s = csub(s, cmul(c[u], x[u]));
+    }
     reinterpret_cast<cplx*>(w)[i] = s;
 }
 
