@@ -100,23 +100,22 @@ __global__ void __launch_bounds__(256) rc_zgemm_batched(GemmArgs g) {
     const cplx* A = reinterpret_cast<const cplx*>(g.A) + (long long)batch * g.strideA;
     const cplx* B = reinterpret_cast<const cplx*>(g.B) + (long long)batch * g.strideB;
     cplx* C = reinterpret_cast<cplx*>(g.C) + (long long)batch * g.strideC;
+    // the epilogue's operands (this thread's element of C and of the added matrix, the weights)
+    // are loaded with the fragments, so their round trip overlaps the product's
+    const int r = 16 * bi + (tid >> 4), c = 16 * bj + (tid & 15);
+    const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * (g.alpha_scale != 0.0 ? g.alpha_scale : 1.0);
+    const double dl = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
+    const cplx cin = g.beta != 0.0 ? C[r * g.Dp + c] : cmake(0, 0);
+    const cplx din = g.addD ? (reinterpret_cast<const cplx*>(g.addD) + (long long)batch * g.strideD)[r * g.Dp + c]
+                            : cmake(0, 0);
     cacc acc;
     cacc_zero(acc);
     tile_mma(acc, A, g.opA, B, g.opB, g.Dp, bi, bj, wave, lane);
     cplx v = reduce4(acc, red, wave, lane);
-    const double al = (g.alpha ? g.alpha[batch * g.alpha_stride] : 1.0) * (g.alpha_scale != 0.0 ? g.alpha_scale : 1.0);
-    const int r = 16 * bi + (tid >> 4), c = 16 * bj + (tid & 15);
     cplx out = cscale(al, v);
-    if (g.beta != 0.0) out = cadd(out, cscale(g.beta, C[r * g.Dp + c]));
-    if (g.addD) {
-        const cplx* Dm = reinterpret_cast<const cplx*>(g.addD) + (long long)batch * g.strideD;
-        const double dl = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
-        out = cadd(out, cscale(dl * g.dscale, Dm[r * g.Dp + c]));
-    }
-    if (g.add_identity != 0.0 && r == c && r < g.D) {
-        const double il = g.dalpha ? g.dalpha[batch * g.alpha_stride] : 1.0;
-        out.re += il * g.add_identity;
-    }
+    if (g.beta != 0.0) out = cadd(out, cscale(g.beta, cin));
+    if (g.addD) out = cadd(out, cscale(dl * g.dscale, din));
+    if (g.add_identity != 0.0 && r == c && r < g.D) out.re += dl * g.add_identity;
     C[r * g.Dp + c] = out;
 }
 

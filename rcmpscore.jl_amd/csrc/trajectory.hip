@@ -53,9 +53,12 @@ __device__ bool wait_flags(const Ctl& ctl, uint32_t* const* addr_lane, int n, ui
         long long spins = 0;
         int bad = 0;
         while (true) {
-            bool ok = a ? (ld_flag(a) >= target) : true;
+            // the flag and the abort word are loaded together: one round trip per poll
+            const uint32_t fv = a ? ld_flag(a) : 0u;
+            const uint32_t ab = ld_flag(ctl.abort_flag);
+            const bool ok = a ? (fv >= target) : true;
             if (__all(ok)) break;
-            if (++spins > ctl.spin_limit || ld_flag(ctl.abort_flag)) { bad = 1; break; }
+            if (++spins > ctl.spin_limit || ab) { bad = 1; break; }
             __builtin_amdgcn_s_sleep(1);
         }
         if (lane == 0) {
