@@ -986,7 +986,10 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
         Sy.has_scalar = kHasScalar[s];
         Sy.adj = s % 2;
         Sy.first_task = (int)tasks.size();
-        Sy.ntasks = (!adjoint && s % 2) || (quartic_only && s < 4) ? 0 : kLive[s] * NT;
+        // the commutator adjoint system (s = 3) forms channel 1's E = X C in NT tasks of its own,
+        // ranked after the channels' tasks, so channel 1 has one level-1 product like every other
+        // channel and channel 0's insertion does not wait for a second one
+        Sy.ntasks = (!adjoint && s % 2) || (quartic_only && s < 4) ? 0 : kLive[s] * NT + (s == 3 ? NT : 0);
         Sy.D = D;
         Sy.n_elems = (double)kLive[s] * D * D + kHasScalar[s];
         Sy.Q = st->Q.d(); Sy.Qh = st->Qh.d(); Sy.R = st->R.d(); Sy.Rh = st->Rh.d(); Sy.C = st->C.d();
@@ -1028,7 +1031,6 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
                             t.edge_coef = s2;
                         } else {
                             t.fixed = term == 0 ? fK1A : fK2A;
-                            t.has_E = term == 1;
                         }
                     } else if (!adj) {
                         if (ch == 0) t.fixed = fQF;
@@ -1038,6 +1040,16 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
                         if (ch < 3) { t.edge_src = ch + 1; t.edge_coef = ch + 1; }
                         else t.fixed = fQA;
                     }
+                    tasks.push_back(t);
+                }
+        if (s == 3 && Sy.ntasks > 0)
+            for (int b = 0; b < NB; ++b)
+                for (int c = 0; c < NB; ++c) {
+                    TrajTask t{};
+                    t.sys = sys_base + s; t.ch = 1; t.b = b; t.c = c; t.adj = 1;
+                    t.edge_src = -1;
+                    t.has_E = 1;
+                    t.e_only = 1;
                     tasks.push_back(t);
                 }
         sys.push_back(Sy);

@@ -23,7 +23,8 @@ struct TrajTask {
     int edge_from_E;    // 1: the insertion reads the E buffer (X * [Q,R]) of edge_src
     int has_E;          // compute E = X * C at level 1 (kinetic commutator, adjoint channel 2)
     int trace_kind;     // 0 none; 1: Re sum X conj(O) over the tile; 2: Re diag of the level-1 tile
-    int pad0;
+    int e_only;         // 1: only form and publish channel ch's E tile each stage (no stage input,
+                        // no samples, an empty error record)
     double edge_coef;
     double trace_coef;
     const double* fixed;     // Dp x Dp complex (row-major, interleaved) or null

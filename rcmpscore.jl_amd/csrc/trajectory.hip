@@ -466,7 +466,8 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
     const TrajSys S = load_scalar<TrajSys>(p.sys, tk.sys);
     const int b = tk.b, c = tk.c, ch = tk.ch, adj = tk.adj;
     const int row = 16 * b + ei, col = 16 * c + ej;
-    const bool real_el = row < S.D && col < S.D;
+    const bool e_only = tk.e_only != 0;                  // uniform over the workgroup
+    const bool real_el = !e_only && row < S.D && col < S.D;
     const Ctl ctl{p.abort_flag, p.spin_limit};
     const uint32_t mat_bytes = (uint32_t)(Dp * Dp * 16);
     const uint32_t stage_stride = (uint32_t)S.nlive * mat_bytes;
@@ -570,6 +571,27 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             sXc = sgX[16 + (lane & 15)];
         }
         lap(1);
+        if (e_only) {
+            // E = X C of channel ch: its row block b times C's column block c, the same product
+            // the channel's own task would form at level 1
+            cplx Ee;
+            if constexpr (SPLIT) {
+                double pe[4] = {0.0, 0.0, 0.0, 0.0};
+                const int s2[4] = {-(sXr[0] + sK4), -(sXr[1] + sK4), -(sXr[2] + sK4), -(sXr[3] + sK4)};
+                rc_split_product(pe, pXr, spa, pK4, spb, s2, lane);
+                Ee = reduce_split(pe, redd, wave, lane);
+            } else {
+                cacc ae;
+                cacc_zero(ae);
+                tile_product<NB>(ae, Xr, Kc, wave, lane);
+                Ee = reduce_tile(ae, red, wave, lane);
+            }
+            st_coh(rE, (uint32_t)si * mat_bytes + el_off, Ee);
+            publish(S.flagE + my_tile, ev);
+            k_out = cmake(0, 0);
+            tr_out = 0.0;
+            return true;
+        }
         // level 1
         cacc acc;
         double part[4] = {0.0, 0.0, 0.0, 0.0};
@@ -697,8 +719,10 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
 
     auto publish_Y = [&](int si, uint32_t ev, cplx v) {
         lap(7);
-        st_coh(rY, (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes + el_off, v);
-        publish(flagY + my_tile, ev);
+        if (!e_only) {   // an E task reads channel ch's stage input; the channel's own task writes it
+            st_coh(rY, (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes + el_off, v);
+            publish(flagY + my_tile, ev);
+        }
         lap(10);
     };
 
@@ -869,7 +893,7 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
                     r = cscale(x, cadd(r, d0));
                     v = cadd(r, y);
                 }
-                st_coh(rS, (uint32_t)next_point * stage_stride + (uint32_t)ch * mat_bytes + el_off, v);
+                if (!e_only) st_coh(rS, (uint32_t)next_point * stage_stride + (uint32_t)ch * mat_bytes + el_off, v);
                 ++next_point;
             }
             next = -2;
