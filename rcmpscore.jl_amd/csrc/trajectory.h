@@ -21,7 +21,7 @@ struct TrajTask {
     int adj;
     int edge_src;       // live channel (same system) whose level-1 tile feeds the insertion, -1 none
     int edge_from_E;    // 1: the insertion reads the E buffer (X * [Q,R]) of edge_src
-    int has_E;          // compute E = X * C at level 1 (kinetic commutator, adjoint channel 2)
+    int has_E;          // load the C = [Q, R] column block of E = X C (the E tasks of the kinetic commutator)
     int trace_kind;     // 0 none; 1: Re sum X conj(O) over the tile; 2: Re diag of the level-1 tile
     int e_only;         // 1: only form and publish channel ch's E tile each stage (no stage input,
                         // no samples, an empty error record)

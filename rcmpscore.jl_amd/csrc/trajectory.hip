@@ -10,7 +10,9 @@
 // GPU package (gpu/src/rcmps_gpu/optimizer/runtime/ode_backend.py:500-586,
 // device_control_backend.py:30-47).  The FSAL stage is evaluated only after acceptance.
 //
-// Work decomposition: one workgroup per (system, live channel, 16x16 output tile).  A stage is
+// Work decomposition: one workgroup per (system, live channel, 16x16 output tile), plus, for the
+// kinetic commutator's adjoint system, one E task per tile forming E = X [Q,R] of channel 1, which
+// channel 0's insertion reads (so no channel forms two level-1 products).  A stage is
 // two levels: level 1 forms the tile of P = R X (forward) or X R (adjoint) and publishes it;
 // level 2 forms Q X + X Q^+ (resp. Q^+ X + X Q) while neighbours' P tiles arrive, then adds
 // P R^+ (resp. R^+ P), the insertion terms and the next stage input.  Workgroups never meet at
@@ -572,8 +574,7 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
         }
         lap(1);
         if (e_only) {
-            // E = X C of channel ch: its row block b times C's column block c, the same product
-            // the channel's own task would form at level 1
+            // E = X C of channel ch: its row block b times C's column block c
             cplx Ee;
             if constexpr (SPLIT) {
                 double pe[4] = {0.0, 0.0, 0.0, 0.0};
@@ -613,20 +614,6 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
         }
         const uint32_t poff = (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes;
         st_coh(rP, poff + el_off, Pe);
-        if (tk.has_E) {
-            cplx Ee;
-            if constexpr (SPLIT) {
-                double pe[4] = {0.0, 0.0, 0.0, 0.0};
-                const int s2[4] = {-(sXr[0] + sK4), -(sXr[1] + sK4), -(sXr[2] + sK4), -(sXr[3] + sK4)};
-                rc_split_product(pe, pXr, spa, pK4, spb, s2, lane);
-                Ee = reduce_split(pe, redd, wave, lane);
-            } else {
-                cacc_zero(acc);
-                tile_product<NB>(acc, Xr, Kc, wave, lane);
-                Ee = reduce_tile(acc, red, wave, lane);
-            }
-            st_coh(rE, (uint32_t)si * mat_bytes + el_off, Ee);
-        }
         double trv = 0.0;
         if (tk.trace_kind == 2 && b == c && ei == ej && real_el) trv = Pe.re;
         if (tk.trace_kind == 1 && real_el) {
@@ -634,7 +621,6 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             trv = x.re * trop_e.re + x.im * trop_e.im;   // Re(x conj(o))
         }
         publish(flagP + my_tile, ev);
-        if (tk.has_E) publish(S.flagE + my_tile, ev);
         lap(2);
         // level 2a
         if constexpr (SPLIT) {
