@@ -58,7 +58,7 @@ def profiled_traffic(wl):
     return None
 
 
-def cpu_baseline(R, Q, rho, wl, gpu_stats, attempts_sample=10):
+def cpu_baseline(R, Q, rho, wl, gpu_stats, attempts_sample=40):
     """Oracle (FP64 C port) timed on a bounded sample: `attempts_sample` DOP853 attempts of each
     of the six trajectories, run concurrently on six host threads; the full evaluation time is
     extrapolated with the attempt/dense-step counts of the GPU run, whose step sequences equal the
@@ -206,7 +206,8 @@ def main(argv=None):
     ap.add_argument("--gemm-variant", type=int, default=None, metavar="V",
                     help="force batched-product tile variant V (0-3) instead of the autotuner's choice "
                          "(a profiler's per-dispatch overhead distorts the tuner's timings)")
-    ap.add_argument("--cpu-attempts", type=int, default=10)
+    ap.add_argument("--cpu-attempts", type=int, default=40,
+                    help="DOP853 attempts per trajectory in the CPU-oracle sample (about 12 s of CPU work)")
     ap.add_argument("--copy-reference", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of this many MiB (a known-size kernel that "
                          "calibrates HBM byte counters in a profile of the same command)")
