@@ -79,6 +79,34 @@ __device__ __forceinline__ void st_flag(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sum of the 256 threads' (re, im) into out[0..1] by the pairwise tree t += t + s, s = 128, 64,
+// ..., 1: the first two levels through LDS, the last six inside wave 0 by shuffles (lane t adds
+// lane t + s's value to its own, the operands and order of the LDS tree).
+__device__ __forceinline__ void tree_sum_256(double re, double im, double* sre, double* sim, double* out) {
+    const int t = threadIdx.x;
+    sre[t] = re;
+    sim[t] = im;
+    __syncthreads();
+    if (t < 128) {
+        sre[t] += sre[t + 128];
+        sim[t] += sim[t + 128];
+    }
+    __syncthreads();
+    if (t < 64) {
+        re = sre[t] + sre[t + 64];
+        im = sim[t] + sim[t + 64];
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) {
+            re += __shfl_down(re, s);
+            im += __shfl_down(im, s);
+        }
+        if (t == 0) {
+            out[0] = re;
+            out[1] = im;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // FP64 MFMA 16x16x4.  Operand layout (one wave): A(i,k) at lane i + 16k, B(k,j) at lane
 // j + 16k; result element (4q + lane/16, lane%16) in component q of the v4d accumulator.

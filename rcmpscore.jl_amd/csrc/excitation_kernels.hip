@@ -30,23 +30,24 @@ __global__ void __launch_bounds__(256) rc_mdots(const double* V, long long sb, l
     const int k = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
     const cplx* v = reinterpret_cast<const cplx*>(V) + (long long)b * sb + (long long)k * sv;
     const cplx* w = reinterpret_cast<const cplx*>(W) + (long long)b * sw;
+    // thread t adds elements t, t + 256, ... in increasing order, eight elements' loads in flight
     double re = 0, im = 0;
-    for (int i = t; i < n; i += 256) {
-        const cplx a = v[i], c = w[i];
-        re += a.re * c.re + a.im * c.im;
-        im += a.re * c.im - a.im * c.re;
+    for (int i0 = t; i0 < n; i0 += 8 * 256) {
+        cplx a[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 256 * u < n) {
+                a[u] = v[i0 + 256 * u];
+                c[u] = w[i0 + 256 * u];
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 256 * u < n) {
+                re += a[u].re * c[u].re + a[u].im * c[u].im;
+                im += a[u].re * c[u].im - a[u].im * c[u].re;
+            }
     }
-    sre[t] = re;
-    sim[t] = im;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) { sre[t] += sre[t + s]; sim[t] += sim[t + s]; }
-        __syncthreads();
-    }
-    if (t == 0) {
-        out[2 * ((long long)b * ldo + k)] = sre[0];
-        out[2 * ((long long)b * ldo + k) + 1] = sim[0];
-    }
+    tree_sum_256(re, im, sre, sim, out + 2 * ((long long)b * ldo + k));
 }
 
 // W_b -= sum_{k < nk} h[b * ldh + k] V_{b,k}; grid (ceil(n / 256), nbases)
@@ -58,7 +59,19 @@ __global__ void rc_msubtract(const double* V, long long sb, long long sv, int nk
     const cplx* hh = reinterpret_cast<const cplx*>(h) + (long long)b * ldh;
     cplx* w = reinterpret_cast<cplx*>(W) + (long long)b * sw;
     cplx s = w[i];
-    for (int k = 0; k < nk; ++k) s = csub(s, cmul(hh[k], v[(long long)k * sv + i]));
+    // eight vectors' loads in flight, subtracted in increasing k
+    for (int k0 = 0; k0 < nk; k0 += 8) {
+        cplx x[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < nk) {
+                x[u] = v[(long long)(k0 + u) * sv + i];
+                c[u] = hh[k0 + u];
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < nk) s = csub(s, cmul(c[u], x[u]));
+    }
     w[i] = s;
 }
 

@@ -389,34 +389,6 @@ __global__ void rc_sylvester_diag(int Dp, int D, int adjoint, double shift, cons
     z[i] = cmake(-q.re, -q.im);
 }
 
-// Sum of the 256 threads' (re, im) into out[0..1] by the pairwise tree t += t + s, s = 128, 64,
-// ..., 1: the first two levels through LDS, the last six inside wave 0 by shuffles (lane t adds
-// lane t + s's value to its own, the operands and order of the LDS tree).
-__device__ __forceinline__ void tree_sum_256(double re, double im, double* sre, double* sim, double* out) {
-    const int t = threadIdx.x;
-    sre[t] = re;
-    sim[t] = im;
-    __syncthreads();
-    if (t < 128) {
-        sre[t] += sre[t + 128];
-        sim[t] += sim[t + 128];
-    }
-    __syncthreads();
-    if (t < 64) {
-        re = sre[t] + sre[t + 64];
-        im = sim[t] + sim[t + 64];
-#pragma unroll
-        for (int s = 32; s > 0; s >>= 1) {
-            re += __shfl_down(re, s);
-            im += __shfl_down(im, s);
-        }
-        if (t == 0) {
-            out[0] = re;
-            out[1] = im;
-        }
-    }
-}
-
 // Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.  Thread t
 // adds elements t, t + 256, ... in increasing order, eight elements' loads issued together.
 __global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride, int n, const double* w,
