@@ -3,6 +3,8 @@
 Bars: energy within 1e-12 relative and gradient within 1e-10 relative (to max |G|) of the oracle
 on the same state and rho0 (the FP64 parity bar); stationary state and boundary solve within
 their solver tolerances; closed forms and the golden fixture as in tests/test_oracle.py.
+The oracle cases span one tile block (D <= 16) and three (D = 40: 3 x 3 tiles per channel, with
+nine workgroups forming the commutator adjoint's E = X [Q,R] apart from its channels).
 """
 import math
 import sys
@@ -57,7 +59,7 @@ def test_stationary_state_matches_oracle(rg, d, seed, scaling):
 
 
 @pytest.mark.parametrize("d,seed,scaling,n_quad", [(4, 7, 0.3, 32), (8, 42, 0.2, 128), (5, 3, 0.4, 64),
-                                                    (16, 11, 0.2, 64)])
+                                                    (16, 11, 0.2, 64), (40, 5, 0.2, 32)])
 def test_fg_phi4_matches_oracle(rg, d, seed, scaling, n_quad):
     note(f"case: fg_phi4 D={d} seed={seed} n_quad={n_quad}")
     s = make_state(rg, d, seed, scaling)
