@@ -114,7 +114,7 @@ def test_resume_reproduces_the_uninterrupted_run(tmp_path):
                                                                          history=3), stationary=stationary))
     first = run(optimize(r, k, rho, e, g, objective, OptimizationControls(max_iterations=4, gradient_tol=1e-12,
                                                                           history=3), stationary=stationary))
-    assert first[-1]["reason"] == "requested iteration limit reached"
+    assert first[-1]["reason"] == "iteration cap max_iterations reached"
     path = save_resume_state(tmp_path / "run" / "state.npz", first[-1]["state"])
     resumed_state = load_resume_state(path)
     assert resumed_state["iteration"] == 4 and len(resumed_state["pairs"]) == 3
@@ -151,5 +151,5 @@ def test_plain_quadratic_converges_and_budget_stops_resumably():
     assert done[-1]["converged"] and done[-1]["state"]["gradient_norm"] <= 1e-10
     budget = run(optimize(r, k, rho, e, g, objective, OptimizationControls(gradient_tol=1e-12, max_seconds=1e-9),
                           stationary=stationary))
-    assert budget[-1]["reason"].startswith("optimization time budget reached")
+    assert budget[-1]["reason"].startswith("time budget max_seconds spent")
     assert budget[-1]["state"]["iteration"] == 0
