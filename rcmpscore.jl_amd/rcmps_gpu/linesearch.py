@@ -56,6 +56,25 @@ class HagerZhangLineSearch:
                   (2 * self.c1 - 1) * origin.slope >= t.slope >= self.c2 * origin.slope)
         return bool(exact), bool(approx)
 
+    def __call__(self, fg, x, p, fg0, initialguess=1.0, *, retract, inner, acceptfirst=False):
+        """OptimKit's HagerZhangLineSearch used directly as a line search (ls_lenient,
+        src/optimization.jl:106): the search's final trial is returned without a re-check."""
+        f0, g0 = fg0
+        slope0 = inner(x, g0, p)
+        if not (math.isfinite(slope0) and slope0 < 0):
+            raise SolverFailure("line search direction is not a descent direction")
+        calls = [0]
+
+        def evaluate(alpha):
+            y, v = retract(x, p, alpha)
+            calls[0] += 1
+            f, g = fg(y)
+            return Trial(alpha, f, inner(y, g, v), (y, g, v))
+
+        t, _ = self.search(evaluate, Trial(0.0, f0, slope0, (x, g0, p)), initialguess, acceptfirst)
+        y, g, v = t.data
+        return y, t.f, g, v, t.alpha, calls[0]
+
     def search(self, evaluate, origin: Trial, alpha: float, acceptfirst: bool = True):
         """evaluate(step) -> Trial.  Returns (final trial, list of trial steps).  With acceptfirst
         (what OptimKit's L-BFGS passes) a first trial meeting either Wolfe condition is returned
