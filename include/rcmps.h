@@ -45,6 +45,7 @@ extern "C" {
 #define RCMPS_ERR_ARGUMENT 11        /* invalid argument (ArgumentError) */
 #define RCMPS_ERR_DEVICE 12          /* HIP runtime error */
 #define RCMPS_ERR_UNSUPPORTED 13     /* configuration not built (e.g. D above the kernel limit) */
+#define RCMPS_ERR_SCRATCH_LIMIT 14   /* input larger than a no-alloc entry point's fixed scratch */
 
 #define RCMPS_FLAG_NO_BOUNDARY 1     /* research switch of evaluate(...; boundary=false) */
 /* Split precision: the D x D products of the trajectory right-hand sides run on the INT8 MFMA
@@ -180,6 +181,18 @@ int rcmps_kinetic(rcmps_context* ctx, const rcmps_state* st, int which, double m
 /* Solve (-L* + |I><rho0|) H = a - tr(rho0 a) I, tr(rho0 H) = 0. */
 int rcmps_boundary_environment(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol,
                                double* h, rcmps_stats* stats);
+
+/* rcmps_boundary_environment without any allocation: the same solve with the same kernels and the
+   same result, its device workspace (GMRES basis, temporaries, device scalars) carved from a
+   library-owned static device array of 4 MiB per GPU (held until the process ends) and its
+   Hessenberg read-back through host stack memory.  Fits every D <= 64 (D = 64 takes 3.2 MiB);
+   a state too large for the array returns RCMPS_ERR_SCRATCH_LIMIT.  All calls on one GPU share its
+   array, so calls that overlap on one GPU race (serialise them, e.g. one context per GPU).  No
+   counterpart in the reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The context must
+   have been created (rcmps_context_create sets the array up on its device). */
+int rcmps_boundary_environment_noalloc(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol,
+                                       double* h, rcmps_stats* stats);
 
 /* After rcmps_fg_phi4 on this context: copy the samples of one trajectory system
    (2 n_quad + 1 points x live channels x D x D, row-major) into a device buffer of

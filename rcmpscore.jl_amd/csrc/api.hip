@@ -13,6 +13,7 @@
 #include <atomic>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -162,6 +163,9 @@ void quadrature(int n, double max_point, std::vector<double>& pts, std::vector<d
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// device address of this device's copy of the no-alloc scratch array (defined at the end of this file)
+hipError_t noalloc_scratch_setup(int device);
+
 }  // namespace
 
 struct rcmps_context {
@@ -196,6 +200,9 @@ struct rcmps_context {
         double us[4] = {0.0, 0.0, 0.0, 0.0};
     };
     int gemm_force = -1;
+    // set inside rcmps_boundary_environment_noalloc: products take the tuned variant or variant 0
+    // (same bits) instead of measuring, since the tuner allocates its operands
+    bool no_alloc = false;
     std::map<std::pair<int, int>, GemmTune> gemm_tuned;
     DevBuf tune_buf;
     // contexts (own stream and workspaces) on which the per-state pullback, boundary and metric
@@ -248,6 +255,7 @@ int gemm_variant(rcmps_context* ctx, int Dp, int nbatch) {
     const auto key = std::make_pair(Dp, gemm_class(nbatch));
     auto it = ctx->gemm_tuned.find(key);
     if (it != ctx->gemm_tuned.end()) return it->second.variant;
+    if (ctx->no_alloc) return 0;
     // a stream under graph capture takes the default without measuring
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return 0;
@@ -301,7 +309,9 @@ void launch_zgemm(rcmps_context* ctx, const GemmArgs& g, int nbatch) {
 struct Dev {
     rcmps_context* ctx;
     int Dp, D;
+    double* sm = nullptr;   // 64 doubles of device scalars; the context's `small` buffer when null
     int n() const { return Dp * Dp; }
+    double* small() const;
     dim3 eb() const { return dim3(ceil_div(n(), 256)); }
 
     void gemm(int opA, const double* A, int opB, const double* B, double* C, double alpha = 1.0,
@@ -370,12 +380,14 @@ struct Dev {
         return cd(h[0], h[1]);
     }
     double norm(const double* x) {
-        double* out = ctx->small.d() + 32;
+        double* out = small() + 32;
         hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, x, 0LL, n(), x, out);
         ck(hipGetLastError());
         return std::sqrt(std::max(0.0, fetch2(out).real()));
     }
 };
+
+double* Dev::small() const { return sm ? sm : ctx->small.d(); }
 
 void pad_in(rcmps_context* ctx, int D, int Dp, const double* src, double* dst) {
     ck(hipMemsetAsync(dst, 0, (size_t)Dp * Dp * 16, ctx->stream));
@@ -462,25 +474,42 @@ void lindblad(Dev& dv, const rcmps_state* st, const double* X, double* out, bool
 // k's Hessenberg column (two pinned slots): the host's rotations overlap the next step's products.
 // A step queued past convergence only fills a basis vector the solution does not use, so the
 // iterates, the results and the matvec count are those of the unpipelined iteration.
+// Buffers of one GMRES solve supplied by the caller instead of the context's: V ((m + 1) vectors), w
+// (one vector), h (gmres_col(m) doubles + 64 bytes) on the device, and two host read-back slots of
+// gmres_col(m) doubles each.
+struct GmresBufs {
+    double *V, *w, *h, *host;
+};
+inline size_t gmres_col(int m) { return (size_t)4 * (m + 1) + 2; }   // h1, h2 (complex), <w,w>
+
 bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const double* b, double* x,
-           double tol_abs, int m, int maxiter, int& matvecs, double& resid) {
+           double tol_abs, int m, int maxiter, int& matvecs, double& resid, const GmresBufs* gb = nullptr) {
     rcmps_context* ctx = dv.ctx;
     const int n = dv.n();
-    const size_t col = (size_t)4 * (m + 1) + 2;   // doubles per read-back: h1, h2 (complex), <w,w>
-    ck(ctx->kry_V.ensure((size_t)(m + 1) * n * 16));
-    ck(ctx->kry_w.ensure((size_t)n * 16));
-    ck(ctx->kry_h.ensure(col * 8 + 64));
-    if (ctx->kry_pinned_doubles < 2 * col) {
-        ck(hipStreamSynchronize(ctx->stream));
-        if (ctx->kry_pinned) ck(hipHostFree(ctx->kry_pinned));
-        ctx->kry_pinned = nullptr;
-        ctx->kry_pinned_doubles = 0;
-        ck(hipHostMalloc(reinterpret_cast<void**>(&ctx->kry_pinned), 2 * col * 8));
-        ctx->kry_pinned_doubles = 2 * col;
+    const size_t col = gmres_col(m);   // doubles per read-back
+    double *V, *w, *hdev, *slots;
+    if (gb) {
+        V = gb->V;
+        w = gb->w;
+        hdev = gb->h;
+        slots = gb->host;
+    } else {
+        ck(ctx->kry_V.ensure((size_t)(m + 1) * n * 16));
+        ck(ctx->kry_w.ensure((size_t)n * 16));
+        ck(ctx->kry_h.ensure(col * 8 + 64));
+        if (ctx->kry_pinned_doubles < 2 * col) {
+            ck(hipStreamSynchronize(ctx->stream));
+            if (ctx->kry_pinned) ck(hipHostFree(ctx->kry_pinned));
+            ctx->kry_pinned = nullptr;
+            ctx->kry_pinned_doubles = 0;
+            ck(hipHostMalloc(reinterpret_cast<void**>(&ctx->kry_pinned), 2 * col * 8));
+            ctx->kry_pinned_doubles = 2 * col;
+        }
+        V = ctx->kry_V.d();
+        w = ctx->kry_w.d();
+        hdev = ctx->kry_h.d();
+        slots = ctx->kry_pinned;
     }
-    double* V = ctx->kry_V.d();
-    double* w = ctx->kry_w.d();
-    double* hdev = ctx->kry_h.d();
     ck(hipMemsetAsync(x, 0, (size_t)n * 16, ctx->stream));
     std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), sn(m), yk(m);
     std::vector<double> cs(m);
@@ -506,8 +535,7 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
         hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hnn);
         hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(k + 1) * n * 2);
         ck(hipGetLastError());
-        ck(hipMemcpyAsync(ctx->kry_pinned + (size_t)(k & 1) * col, hdev, col * 8, hipMemcpyDeviceToHost,
-                          ctx->stream));
+        ck(hipMemcpyAsync(slots + (size_t)(k & 1) * col, hdev, col * 8, hipMemcpyDeviceToHost, ctx->stream));
         ck(hipEventRecord(ctx->kry_ev[k & 1], ctx->stream));
     };
     double beta = residual_into_w();
@@ -527,7 +555,7 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
             if (k + 1 < m) queue_step(k + 1);
             ck(hipEventSynchronize(ctx->kry_ev[k & 1]));
             ++matvecs;
-            const cd* hv = reinterpret_cast<const cd*>(ctx->kry_pinned + (size_t)(k & 1) * col);
+            const cd* hv = reinterpret_cast<const cd*>(slots + (size_t)(k & 1) * col);
             for (int i = 0; i <= k; ++i)
                 H[(size_t)i * m + k] += hv[i] + hv[(m + 1) + i];
             double hn = std::sqrt(std::max(0.0, hv[2 * (m + 1)].real()));
@@ -662,6 +690,7 @@ const char* rcmps_status_string(int s) {
         case RCMPS_ERR_ARGUMENT: return "invalid_argument";
         case RCMPS_ERR_DEVICE: return "device_error";
         case RCMPS_ERR_UNSUPPORTED: return "unsupported";
+        case RCMPS_ERR_SCRATCH_LIMIT: return "scratch_limit";
         default: return "unknown";
     }
 }
@@ -682,6 +711,7 @@ int rcmps_context_create(int device, rcmps_context** out) {
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
             ctx->clock_mhz = khz / 1000.0;
         ck(ctx->abort_word.ensure(256));
+        ck(noalloc_scratch_setup(device));
         // the LDS-staged product variant takes up to 66 KB of dynamic LDS at Dp = 64
         ck(hipFuncSetAttribute(reinterpret_cast<const void*>(rc_zgemm_batched_lds),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)gemm_lds_bytes(64)));
@@ -839,24 +869,34 @@ namespace {
 // boundary Poisson solve on padded device matrices: h solves (-L* + |I><rho|) h = a - tr(rho a) I
 // GMRES aims at `tol`; the solution is accepted when the true residual is within 20 accept_tol
 // (relative to max(1, |b|)), the bar of src/transfer.jl:140-152.
+// Device buffers of one boundary solve supplied by the caller instead of the context's: five
+// matrices t1, t2, z, b, x, two matrices at tmp, 64 doubles at small, and the GMRES buffers.
+struct BoundaryBufs {
+    double *t1, *t2, *z, *b, *x, *tmp, *small;
+    GmresBufs g;
+};
+
 void boundary_solve(rcmps_context* ctx, const rcmps_state* st, const double* a, double tol, double accept_tol,
-                    double* h, rcmps_stats* stats) {
+                    double* h, rcmps_stats* stats, const BoundaryBufs* bb = nullptr) {
     const int D = st->D, Dp = st->Dp;
     const size_t mb = (size_t)Dp * Dp * 16;
     Dev dv{ctx, Dp, D};
-    ck(ctx->kry_t1.ensure(mb));
-    ck(ctx->kry_t2.ensure(mb));
-    ck(ctx->kry_z.ensure(mb));
-    ck(ctx->kry_b.ensure(mb));
-    ck(ctx->kry_x.ensure(mb));
-    ck(ctx->kry_tmp.ensure(2 * mb));
-    double* t1 = ctx->kry_t1.d();
-    double* t2 = ctx->kry_t2.d();
-    double* z = ctx->kry_z.d();
-    double* b = ctx->kry_b.d();
-    double* tmp = ctx->kry_tmp.d();
+    double *t1, *t2, *z, *b, *x, *tmp;
+    if (bb) {
+        t1 = bb->t1; t2 = bb->t2; z = bb->z; b = bb->b; x = bb->x; tmp = bb->tmp;
+        dv.sm = bb->small;
+    } else {
+        ck(ctx->kry_t1.ensure(mb));
+        ck(ctx->kry_t2.ensure(mb));
+        ck(ctx->kry_z.ensure(mb));
+        ck(ctx->kry_b.ensure(mb));
+        ck(ctx->kry_x.ensure(mb));
+        ck(ctx->kry_tmp.ensure(2 * mb));
+        t1 = ctx->kry_t1.d(); t2 = ctx->kry_t2.d(); z = ctx->kry_z.d(); b = ctx->kry_b.d(); x = ctx->kry_x.d();
+        tmp = ctx->kry_tmp.d();
+    }
     double* tmp2 = tmp + (size_t)Dp * Dp * 2;
-    double* tr2 = ctx->small.d();
+    double* tr2 = dv.small();
     if (D == 1) {
         ck(hipMemsetAsync(h, 0, mb, ctx->stream));
         return;
@@ -883,8 +923,8 @@ void boundary_solve(rcmps_context* ctx, const rcmps_state* st, const double* a, 
     int mv = 0;
     double res = 0;
     const int m = std::min(D * D, 40);
-    bool ok = gmres(dv, A, b, ctx->kry_x.d(), tol_abs, m, 200, mv, res);
-    pre(ctx->kry_x.d(), h);
+    bool ok = gmres(dv, A, b, x, tol_abs, m, 200, mv, res, bb ? &bb->g : nullptr);
+    pre(x, h);
     // true residual of the physical equation (src/transfer.jl:148-152)
     op(h, tmp);
     dv.axpby(cd(1, 0), tmp, cd(-1, 0), b, tmp);
@@ -2062,3 +2102,86 @@ int rcmps_trajectory_samples(rcmps_context* ctx, int system, double* dst, long l
 }  // extern "C"
 
 #include "excitation.inc"
+
+// ==========================================================================================
+// rcmps_boundary_environment_noalloc: the boundary Poisson solve of rcmps_boundary_environment on a
+// fixed device array instead of allocated workspaces, for callers that cannot allocate.  Every
+// call on one device shares that device's array: calls that overlap on one device race.
+// ==========================================================================================
+#define RC_NOALLOC_SCRATCH_BYTES ((size_t)4 << 20)
+#define RC_NOALLOC_MAX_DEVICES 64
+
+// Layout per call (mb = 16 Dp^2 bytes, m = min(D^2, 40) GMRES vectors): padded input and output
+// (2 mb), t1, t2, z, b, x (5 mb), tmp (2 mb), the basis V ((m + 1) mb), w (mb), the Hessenberg column
+// h, then 64 doubles of device scalars.  Everything a kernel reads is written earlier in the same
+// call (pad_in and GMRES clear what has to start at zero), so stale contents never matter.
+static __device__ __align__(256) double scratch[RC_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[RC_NOALLOC_MAX_DEVICES] = {};
+
+hipError_t noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= RC_NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    // the current device is `device` (rcmps_context_create set it): its copy of the array
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+    return hipSuccess;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// bytes of scratch one boundary solve at padded size Dp with m GMRES vectors takes
+size_t noalloc_boundary_bytes(int Dp, int m) {
+    const size_t mb = (size_t)Dp * Dp * 16;
+    return (size_t)(2 + 5 + 2 + (m + 1) + 1) * mb + align256(gmres_col(m) * 8 + 64) + 64 * 8;
+}
+
+struct NoAllocScope {
+    rcmps_context* ctx;
+    explicit NoAllocScope(rcmps_context* c) : ctx(c) { ctx->no_alloc = true; }
+    ~NoAllocScope() { ctx->no_alloc = false; }
+};
+
+}  // namespace
+
+extern "C" int rcmps_boundary_environment_noalloc(rcmps_context* ctx, const rcmps_state* st, const double* a,
+                                                  double tol, double* h, rcmps_stats* stats) {
+    if (!ctx || !st || !a || !h || !(tol > 0)) return RCMPS_ERR_ARGUMENT;
+    const int D = st->D, Dp = st->Dp;
+    const int m = std::min(D * D, 40);
+    if (noalloc_boundary_bytes(Dp, m) > RC_NOALLOC_SCRATCH_BYTES) return RCMPS_ERR_SCRATCH_LIMIT;
+    double* base = (ctx->device >= 0 && ctx->device < RC_NOALLOC_MAX_DEVICES) ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!base) return RCMPS_ERR_DEVICE;
+    // host read-back slots of the GMRES Hessenberg columns (pageable: nothing is allocated)
+    double slots[2 * 166];
+    static_assert(2 * 166 >= 2 * 4 * (40 + 1) + 4, "two GMRES columns at m = 40");
+    NoAllocScope scope(ctx);
+    return guard([&] {
+        const size_t md = (size_t)Dp * Dp * 2;   // doubles per padded matrix
+        double* p = base;
+        double* ap = p; p += md;
+        double* hp = p; p += md;
+        BoundaryBufs bb{};
+        bb.t1 = p; p += md;
+        bb.t2 = p; p += md;
+        bb.z = p; p += md;
+        bb.b = p; p += md;
+        bb.x = p; p += md;
+        bb.tmp = p; p += 2 * md;
+        bb.g.V = p; p += (size_t)(m + 1) * md;
+        bb.g.w = p; p += md;
+        bb.g.h = p; p += align256(gmres_col(m) * 8 + 64) / 8;
+        bb.small = p;
+        bb.g.host = slots;
+        pad_in(ctx, D, Dp, a, ap);
+        boundary_solve(ctx, st, ap, tol, tol, hp, stats, &bb);
+        pad_out(ctx, D, Dp, hp, h);
+        ck(hipStreamSynchronize(ctx->stream));
+    });
+}

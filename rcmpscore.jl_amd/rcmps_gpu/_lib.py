@@ -20,6 +20,7 @@ RCMPS_OK = 0
 STATUS = {
     0: "success", 1: "max_steps", 2: "step_underflow", 3: "nonfinite_error", 5: "missing_samples",
     7: "wait_limit", 10: "solver_failure", 11: "invalid_argument", 12: "device_error", 13: "unsupported",
+    14: "scratch_limit",
 }
 SYSTEMS = ("kinetic forward", "kinetic adjoint", "kinetic-commutator forward",
            "kinetic-commutator adjoint", "phi4 forward", "phi4 adjoint")
@@ -108,6 +109,7 @@ SIGNATURES = {
     "rcmps_fg_phi4_rk": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.POINTER(Integrator), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double), _P, _P, ctypes.POINTER(Stats)]),
     "rcmps_boundary_environment": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Stats)]),
+    "rcmps_boundary_environment_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, ctypes.POINTER(Stats)]),
     "rcmps_trajectory_samples": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_longlong]),
     "rcmps_excitation_create": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_P)]),
     "rcmps_excitation_destroy": (None, [_P]),
@@ -203,6 +205,8 @@ def check(status: int, what: str, stats: Stats | None = None):
         raise ValueError(f"{what}: invalid argument")
     if status == 13:
         raise NotImplementedError(f"{what}: configuration not supported by the built kernels")
+    if status == 14:
+        raise ValueError(f"{what}: input larger than the no-alloc entry point's fixed scratch")
     raise RuntimeError(f"{what}: {STATUS.get(status, status)}")
 
 

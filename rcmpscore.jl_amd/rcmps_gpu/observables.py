@@ -145,8 +145,11 @@ def fg_phi4_rk(s: State, g: float, *, gradient: bool = True, boundary: bool = Tr
     return e.value, GR, GK, stats.as_dict()
 
 
-def boundary_environment(s: State, a, *, tol: float = 1e-11) -> np.ndarray:
-    """Solve (-L* + |I><rho|) H = a - tr(rho a) I (src/transfer.jl:122-153)."""
+def boundary_environment(s: State, a, *, tol: float = 1e-11, no_alloc: bool = False) -> np.ndarray:
+    """Solve (-L* + |I><rho|) H = a - tr(rho a) I (src/transfer.jl:122-153).
+
+    no_alloc=True runs the same solve through rcmps_boundary_environment_noalloc, whose device
+    workspace is a fixed library-owned array (D <= 64; larger states raise ValueError)."""
     lib = _lib.load()
     ctx = _lib.context(s.device)
     dev = torch.device("cuda", s.device)
@@ -154,8 +157,9 @@ def boundary_environment(s: State, a, *, tol: float = 1e-11) -> np.ndarray:
     torch.cuda.synchronize(dev)
     h = torch.empty_like(at)
     stats = _lib.Stats()
-    _lib.check(lib.rcmps_boundary_environment(ctx, s.handle, at.data_ptr(), float(tol), h.data_ptr(),
-                                              ctypes.byref(stats)), "boundary_environment", stats)
+    fn = lib.rcmps_boundary_environment_noalloc if no_alloc else lib.rcmps_boundary_environment
+    _lib.check(fn(ctx, s.handle, at.data_ptr(), float(tol), h.data_ptr(), ctypes.byref(stats)),
+               "boundary_environment", stats)
     return h.cpu().numpy()
 
 
