@@ -264,8 +264,29 @@ def record_of_state(s, *, energy=None, gradient=None, symmetry=None, producer="r
         source=source, info=s.info or None)
 
 
-def rcmps_store(s, path, *, overwrite: bool = True, **kw) -> Path:
-    """Store a State as a portable directory (the format shared with the Julia package)."""
+def jld2_fields(s) -> dict:
+    """The schema-1 checkpoint fields of a State (src/storage.jl:10-16): e0 NaN and a zero gradient
+    for a state never evaluated, as State(R, K) initialises them (src/state.jl:52).  An empty info
+    string is stored as "rcmps_gpu" (the JLD2 encoding of an empty String is not reproduced)."""
+    ip, mp = s.integrator_param, s.manifold_param
+    e0 = s.e0 if s.e0 is not None and math.isfinite(s.e0) else float("nan")
+    grad = s.grad if getattr(s, "grad", None) is not None else np.zeros((s.D, s.D), np.complex128)
+    return dict(R=s.R, K=s.K, grad=grad, model=s.model_param.name, coupling=complex(s.model_param.coupling),
+                e0=e0, info=s.info or "rcmps_gpu", integration=(ip.n_quad, ip.max_point, ip.atol, ip.rtol, ip.maxiters),
+                manifold=(mp.metric_reg, mp.precond_reg_prefactor, mp.precond_reg))
+
+
+def rcmps_store(s, path=None, *, directory="data", id=None, overwrite: bool = True, **kw) -> Path:
+    """Store a State (src/storage.jl:4-26): a path ending in .jld2 (or no path: the reference's name
+    "<model>_g<coupling>_D<D>_<id>.jld2" in `directory`) gets a JLD2 schema-1 checkpoint; any other
+    path a portable directory (the format shared with the Julia package)."""
+    if path is None:
+        import secrets
+        tag = id if id is not None else secrets.token_hex(5)
+        path = Path(directory) / f"{s.model_param.name}_g{float(np.real(s.model_param.coupling))}_D{s.D}_{tag}.jld2"
+    if str(path).endswith(".jld2"):
+        from .jld2 import write_jld2
+        return write_jld2(path, jld2_fields(s))
     return write_portable(path, record_of_state(s, **kw), overwrite=overwrite)
 
 

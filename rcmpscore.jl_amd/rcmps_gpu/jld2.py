@@ -1,4 +1,4 @@
-"""JLD2 checkpoints (rcmps_store schema 1, src/storage.jl:4-38) read without h5py.
+"""JLD2 checkpoints (rcmps_store schema 1, src/storage.jl:4-38) read and written without h5py.
 
 JLD2 files are HDF5 files written by Julia's JLD2 package (here "HDF5-based Julia Data Format,
 version 0.2.0").  This module reads the subset of the HDF5 file format that such a checkpoint
@@ -14,6 +14,11 @@ uses, following the published HDF5 File Format Specification (version 3.0):
 read_jld2 returns a StateRecord like the reference GPU package's reader (gpu/src/rcmps_gpu/io.py:
 261-298): R, K, gradient as row-major numpy arrays (HDF5 stores Julia's column-major arrays with
 reversed dimensions), e0 (None when not finite), model, coupling, integration and manifold.
+
+write_jld2 writes a checkpoint in the layout JLD2 0.2 gives it (object order, committed Julia
+types with their julia_type / field_names / field_types attributes, global heap, checksums), so
+that Julia's rcmps_load reads it: rewriting the reference's three checkpoints from their own
+contents reproduces each file byte for byte after the preamble (which names the Julia version).
 """
 from __future__ import annotations
 
@@ -247,3 +252,404 @@ def read_jld2(path) -> StateRecord:
                        gradient=np.ascontiguousarray(g, dtype=np.complex128) if evaluated and g is not None else None,
                        energy=e0 if evaluated else None, integration=integration, manifold=manifold,
                        info=str(f["info"]) if "info" in f else None, source=str(path))
+
+
+# =============================================================================================
+# Writer: the rcmps_store schema-1 checkpoint (src/storage.jl:4-24) in the layout JLD2 0.2 writes
+# for it, so that Julia's rcmps_load reads it back.  Objects follow one another in the order JLD2
+# creates them (its 512-byte preamble, a version-2 superblock, then version-2 object headers with
+# lookup3 checksums, one 4096-byte global heap collection for variable-length data, contiguous data
+# blocks 8-byte aligned):
+#   schema_version (Int64), R and K (Matrix{ComplexF64}: the committed DataType and Complex types,
+#   the Float64 parameter stored as a DataType value), model (String), coupling (ComplexF64), e0
+#   (Float64), grad, info (String), integration and manifold (NamedTuples: committed NamedTuple,
+#   Symbol, Tuple{Symbol...} and Tuple{...} types with their julia_type / field_names /
+#   field_types attributes), the _types group of the nine committed types, the root group.
+# Addresses are relative to the superblock at byte 512; the superblock's end-of-file field holds the
+# file length.
+# =============================================================================================
+PREAMBLE = b"HDF5-based Julia Data Format, version 0.2.0\x00 (Julia 1.12.5 64-bit LE)"
+_BASE = 512
+_GCOL_SIZE = 4096
+_UNDEF = 0xFFFFFFFFFFFFFFFF
+
+
+def lookup3(data: bytes, initval: int = 0) -> int:
+    """Bob Jenkins' lookup3 hashlittle (2006), the checksum of HDF5 version-2 metadata."""
+    mask = 0xFFFFFFFF
+
+    def rot(x, k):
+        return ((x << k) | (x >> (32 - k))) & mask
+
+    n = len(data)
+    a = b = c = (0xDEADBEEF + n + initval) & mask
+    i = 0
+    while n > 12:
+        a = (a + int.from_bytes(data[i:i + 4], "little")) & mask
+        b = (b + int.from_bytes(data[i + 4:i + 8], "little")) & mask
+        c = (c + int.from_bytes(data[i + 8:i + 12], "little")) & mask
+        a = (a - c) & mask; a ^= rot(c, 4); c = (c + b) & mask
+        b = (b - a) & mask; b ^= rot(a, 6); a = (a + c) & mask
+        c = (c - b) & mask; c ^= rot(b, 8); b = (b + a) & mask
+        a = (a - c) & mask; a ^= rot(c, 16); c = (c + b) & mask
+        b = (b - a) & mask; b ^= rot(a, 19); a = (a + c) & mask
+        c = (c - b) & mask; c ^= rot(b, 4); b = (b + a) & mask
+        n -= 12
+        i += 12
+    if n == 0:
+        return c
+    tail = data[i:] + bytes(12 - n)
+    a = (a + int.from_bytes(tail[0:4], "little")) & mask
+    b = (b + int.from_bytes(tail[4:8], "little")) & mask
+    c = (c + int.from_bytes(tail[8:12], "little")) & mask
+    c ^= b; c = (c - rot(b, 14)) & mask
+    a ^= c; a = (a - rot(c, 11)) & mask
+    b ^= a; b = (b - rot(a, 25)) & mask
+    c ^= b; c = (c - rot(b, 16)) & mask
+    a ^= c; a = (a - rot(c, 4)) & mask
+    b ^= a; b = (b - rot(a, 14)) & mask
+    c ^= b; c = (c - rot(b, 24)) & mask
+    return c
+
+
+# ---- datatype encodings (HDF5 datatype message, version 3) ----------------------------------
+_T_I64 = bytes.fromhex("3008000008000000" "00004000")
+_T_F64 = bytes.fromhex("31203f0008000000" "000040" "00340b0034ff030000")
+_T_SYMBOL = bytes.fromhex("3911010010000000" "300000000100000000000800")   # vlen UTF-8 string
+_T_REF = bytes.fromhex("3700000008000000")                                  # object reference
+
+
+def _t_string(n: int) -> bytes:
+    return bytes.fromhex("33110000") + struct.pack("<I", n)
+
+
+def _t_compound(size: int, members) -> bytes:
+    """Version-3 compound: member name (null-terminated), offset in the smallest width, type."""
+    width = 1 if size < 1 << 8 else 2 if size < 1 << 16 else 3 if size < 1 << 24 else 4
+    out = bytes([0x36]) + struct.pack("<H", len(members)) + b"\x00" + struct.pack("<I", size)
+    for name, off, t in members:
+        out += name.encode() + b"\x00" + off.to_bytes(width, "little") + t
+    return out
+
+
+_T_DATATYPE = _t_compound(32, [("name", 0, _T_SYMBOL),
+                               ("parameters", 16, bytes.fromhex("3900000010000000") + _T_REF)])
+_T_COMPLEX = _t_compound(16, [("re", 0, _T_F64), ("im", 8, _T_F64)])
+
+
+def _shared(addr: int) -> bytes:
+    return bytes([3, 2]) + struct.pack("<Q", addr)
+
+
+_SCALAR = bytes.fromhex("02000000")
+
+
+def _simple(*dims) -> bytes:
+    return bytes([2, len(dims), 0, 1]) + b"".join(struct.pack("<Q", d) for d in dims)
+
+
+class _Writer:
+    def __init__(self):
+        self.buf = bytearray()          # content at relative addresses 0..
+        self.gcol = None                # relative address of the global heap collection
+        self.gobjs = []                 # its objects (bytes)
+        self.types = []                 # (name, address) of committed datatypes
+
+    # -- space ---------------------------------------------------------------------------
+    def at(self, addr: int, data: bytes):
+        end = addr + len(data)
+        if len(self.buf) < end:
+            self.buf.extend(bytes(end - len(self.buf)))
+        self.buf[addr:end] = data
+
+    def cursor(self) -> int:
+        return len(self.buf)
+
+    def reserve(self, n: int) -> int:
+        a = self.cursor()
+        self.buf.extend(bytes(n))
+        return a
+
+    def align8(self):
+        self.reserve((-self.cursor()) % 8)
+
+    # -- global heap -----------------------------------------------------------------------
+    def heap(self, payload: bytes) -> bytes:
+        """Store payload as the next global heap object; its 12-byte heap ID (collection, index)."""
+        if self.gcol is None:
+            self.align8()
+            self.gcol = self.reserve(_GCOL_SIZE + 16)
+        self.gobjs.append(payload)
+        return struct.pack("<QI", self.gcol, len(self.gobjs))
+
+    def vlen(self, payload: bytes, count: int) -> bytes:
+        return struct.pack("<I", count) + self.heap(payload)
+
+    def symbol(self, text: str) -> bytes:
+        b = text.encode()
+        return self.vlen(b, len(b))
+
+    def refs(self, addrs) -> bytes:
+        if not addrs:
+            return bytes(16)
+        return self.vlen(b"".join(struct.pack("<Q", a) for a in addrs), len(addrs))
+
+    def flush_heap(self):
+        if self.gcol is None:
+            return
+        body = bytearray()
+        for i, obj in enumerate(self.gobjs, 1):
+            body += struct.pack("<HH4xQ", i, 1, len(obj)) + obj + bytes((-len(obj)) % 8)
+        free = _GCOL_SIZE - 16 - len(body)
+        body += struct.pack("<HH4xQ", 0, 0, free)
+        self.at(self.gcol, b"GCOL\x01\x00\x00\x00" + struct.pack("<Q", _GCOL_SIZE) + bytes(body))
+
+    # -- object headers -------------------------------------------------------------------
+    @staticmethod
+    def header(messages) -> bytes:
+        """Version-2 object header of (type, flags, body) messages, with its checksum."""
+        body = b"".join(bytes([t]) + struct.pack("<H", len(b)) + bytes([f]) + b for t, f, b in messages)
+        flags = 0 if len(body) < 256 else 1
+        head = b"OHDR\x02" + bytes([flags]) + len(body).to_bytes(1 << flags, "little") + body
+        return head + struct.pack("<I", lookup3(head))
+
+    @staticmethod
+    def attribute(name: str, dtype: bytes, space: bytes, data: bytes, shared: bool) -> tuple:
+        nm = name.encode() + b"\x00"
+        body = bytes([2, 1 if shared else 0]) + struct.pack("<HHH", len(nm), len(dtype), len(space)) + nm + dtype + space
+        return (0x0C, 0, body + data)
+
+    def dataset(self, space: bytes, dtype: bytes, shared: bool, layout: bytes) -> int:
+        msgs = [(0x05, 0, b"\x03\x09"), (0x01, 0, space), (0x03, 3 if shared else 1, dtype), (0x08, 0, layout),
+                (0x00, 0, bytes(16))]
+        h = self.header(msgs)
+        a = self.reserve(len(h))
+        self.at(a, h)
+        return a
+
+    def compact(self, space: bytes, dtype: bytes, shared: bool, data: bytes) -> int:
+        return self.dataset(space, dtype, shared, b"\x04\x00" + struct.pack("<H", len(data)) + data)
+
+    def contiguous(self, space: bytes, dtype: bytes, shared: bool, data: bytes) -> int:
+        size = len(self.header([(0x05, 0, b""), (0x01, 0, space), (0x03, 0, dtype), (0x08, 0, bytes(18)),
+                                (0x00, 0, bytes(16))])) + 2
+        hdr = self.cursor()
+        start = hdr + size
+        start += (-start) % 8
+        a = self.dataset(space, dtype, shared, b"\x04\x01" + struct.pack("<QQ", start, len(data)))
+        assert a == hdr
+        self.align8()
+        assert self.cursor() == start
+        self.at(start, data)
+        return a
+
+    def committed(self, dtype: bytes, build) -> int:
+        """Reserve the header of a committed datatype at the cursor, run build(address) -> list of
+        attribute messages (which may write further objects after the header), then fill it in."""
+        a = self.cursor()
+        # the header's size is known before its attribute values are: they are fixed-size records
+        probe = build(None)
+        h_len = len(self.header([(0x03, 0x40, dtype)] + probe))
+        self.reserve(h_len)
+        msgs = build(a)
+        h = self.header([(0x03, 0x40, dtype)] + msgs)
+        assert len(h) == h_len
+        self.at(a, h)
+        return a
+
+    def group(self, links) -> int:
+        msgs = [(0x02, 0, b"\x00\x00" + struct.pack("<QQ", _UNDEF, _UNDEF)), (0x0A, 0, b"\x00\x00")]
+        for name, addr in links:
+            nm = name.encode()
+            msgs.append((0x06, 0, bytes([1, 0x10, 1, len(nm)]) + nm + struct.pack("<Q", addr)))
+        msgs.append((0x00, 0, bytes(16)))
+        h = self.header(msgs)
+        a = self.reserve(len(h))
+        self.at(a, h)
+        return a
+
+
+class _Types:
+    """The committed datatypes of a schema-1 checkpoint, created on first use in JLD2's order."""
+
+    def __init__(self, w: _Writer):
+        self.w = w
+        self.datatype = None
+        self.complex = None
+        self.symbol = None
+
+    def _julia_type(self, name: str, params, target):
+        """julia_type attribute: a DataType record (name, parameter references) whose type is the
+        committed DataType type (itself, for the DataType type)."""
+        w = self.w
+        if target is None:   # size probe: the record is 32 bytes whatever it holds
+            return self.w.attribute("julia_type", _shared(0), _SCALAR, bytes(32), True)
+        rec = w.symbol(name) + w.refs(params() if callable(params) else params)
+        return w.attribute("julia_type", _shared(self.datatype if self.datatype is not None else target),
+                           _SCALAR, rec, True)
+
+    def _datatype_value(self, name: str) -> int:
+        """A DataType value (e.g. Core.Float64 as a type parameter) stored as a scalar dataset."""
+        w = self.w
+        return w.compact(_SCALAR, _shared(self.datatype), True, w.symbol(name) + bytes(16))
+
+    def ensure_datatype(self):
+        if self.datatype is None:
+            w = self.w
+
+            def build(a):
+                return [self._julia_type("Core.DataType", [], a)]
+            self.datatype = w.committed(_T_DATATYPE, build)
+            w.types.append(self.datatype)
+
+    def ensure_complex(self) -> int:
+        self.ensure_datatype()
+        if self.complex is None:
+            w = self.w
+
+            def build(a):
+                if a is None:
+                    return [self._julia_type("", [], None)]
+                name = w.symbol("Base.Complex")
+                param = self._datatype_value("Core.Float64")
+                rec = name + w.refs([param])
+                return [w.attribute("julia_type", _shared(self.datatype), _SCALAR, rec, True)]
+            self.complex = w.committed(_T_COMPLEX, build)
+            w.types.append(self.complex)
+        return self.complex
+
+    def ensure_symbol(self) -> int:
+        if self.symbol is None:
+            w = self.w
+            self.symbol = w.committed(_T_SYMBOL, lambda a: [self._julia_type("Core.Symbol", [], a)])
+            w.types.append(self.symbol)
+        return self.symbol
+
+    def symbol_tuple(self, n: int) -> int:
+        """Tuple{Symbol, ..., Symbol} (n fields "1".."n")."""
+        w = self.w
+        sym = self.ensure_symbol()
+        dtype = _t_compound(16 * n, [(str(i + 1), 16 * i, _T_SYMBOL) for i in range(n)])
+
+        def build(a):
+            if a is None:
+                return [self._julia_type("", [], None),
+                        w.attribute("field_names", _T_SYMBOL, _simple(n), bytes(16 * n), False),
+                        w.attribute("field_types", _T_REF, _simple(n), bytes(8 * n), False)]
+            jt = self._julia_type("Tuple", [sym] * n, a)
+            names = b"".join(w.symbol(str(i + 1)) for i in range(n))
+            return [jt, w.attribute("field_names", _T_SYMBOL, _simple(n), names, False),
+                    w.attribute("field_types", _T_REF, _simple(n), struct.pack(f"<{n}Q", *([sym] * n)), False)]
+        a = w.committed(dtype, build)
+        w.types.append(a)
+        return a
+
+    def value_tuple(self, kinds, julia_name: str) -> int:
+        """Tuple{kinds...} of Int64 ("i") / Float64 ("f") fields "1".."n", each parameter a DataType
+        value written after the type (one per distinct kind, in first-use order)."""
+        w = self.w
+        members = [(str(i + 1), 8 * i, _T_I64 if k == "i" else _T_F64) for i, k in enumerate(kinds)]
+        dtype = _t_compound(8 * len(kinds), members)
+
+        def build(a):
+            if a is None:
+                return [self._julia_type("", [], None)]
+            name = w.symbol(julia_name)
+            seen = {}
+            for k in kinds:
+                if k not in seen:
+                    seen[k] = self._datatype_value("Core.Int64" if k == "i" else "Core.Float64")
+            rec = name + w.refs([seen[k] for k in kinds])
+            return [w.attribute("julia_type", _shared(self.datatype), _SCALAR, rec, True)]
+        a = w.committed(dtype, build)
+        w.types.append(a)
+        return a
+
+    def named_tuple(self, fields_, tuple_name: str) -> int:
+        """NamedTuple{names, Tuple{types}} for fields (name, kind); returns the committed type."""
+        w = self.w
+        n = len(fields_)
+        members = [(nm, 8 * i, _T_I64 if k == "i" else _T_F64) for i, (nm, k) in enumerate(fields_)]
+        dtype = _t_compound(8 * n, members)
+
+        def build(a):
+            if a is None:
+                return [self._julia_type("", [], None)]
+            name = w.symbol("Core.NamedTuple")
+            names_type = self.symbol_tuple(n)
+            names_value = w.compact(_SCALAR, _shared(names_type), True, b"".join(w.symbol(nm) for nm, _ in fields_))
+            types_type = self.value_tuple([k for _, k in fields_], tuple_name)
+            rec = name + w.refs([names_value, types_type])
+            return [w.attribute("julia_type", _shared(self.datatype), _SCALAR, rec, True)]
+        a = w.committed(dtype, build)
+        w.types.append(a)
+        return a
+
+
+def write_jld2(path, fields_: dict) -> Path:
+    """Write a schema-1 checkpoint.  fields_: R, K, grad (Julia matrices as row-major numpy arrays,
+    complex128), model (str), coupling (complex), e0 (float; NaN for a state never evaluated), info
+    (non-empty str), integration (n_quad, max_point, atol, rtol, maxiters), manifold (metric_reg,
+    precond_reg_prefactor, precond_reg).  The file is written through a temporary file and an
+    atomic rename."""
+    import os
+    import tempfile
+
+    R = np.asarray(fields_["R"], dtype=np.complex128)
+    D = R.shape[0]
+    info = str(fields_["info"])
+    if not info:
+        raise ValueError("info must be a non-empty string")
+    w = _Writer()
+    t = _Types(w)
+    w.reserve(48)   # superblock
+
+    def matrix(a):
+        a = np.asarray(a, dtype=np.complex128)
+        if a.shape != (D, D):
+            raise ValueError("R, K and grad must be D x D")
+        # Julia's column-major D x D matrix = the row-major transpose's bytes
+        return np.ascontiguousarray(a.T).astype("<c16").tobytes()
+
+    root = []
+    root.append(("schema_version", w.compact(_SCALAR, _T_I64, False, struct.pack("<q", 1))))
+    cplx = None
+    for key, name in (("R", "R"), ("K", "K")):
+        if cplx is None:
+            cplx = t.ensure_complex()
+        root.append((name, w.contiguous(_simple(D, D), _shared(cplx), True, matrix(fields_[key]))))
+    model = str(fields_["model"]).encode()
+    root.append(("model", w.compact(_SCALAR, _t_string(len(model)), False, model)))
+    c = complex(fields_["coupling"])
+    root.append(("coupling", w.compact(_SCALAR, _shared(cplx), True, struct.pack("<dd", c.real, c.imag))))
+    root.append(("e0", w.compact(_SCALAR, _T_F64, False, struct.pack("<d", float(fields_["e0"])))))
+    root.append(("grad", w.contiguous(_simple(D, D), _shared(cplx), True, matrix(fields_["grad"]))))
+    ib = info.encode()
+    root.append(("info", w.compact(_SCALAR, _t_string(len(ib)), False, ib)))
+    n_quad, max_point, atol, rtol, maxiters = fields_["integration"]
+    nt = t.named_tuple([("n_quad", "i"), ("max_point", "f"), ("atol", "f"), ("rtol", "f"), ("maxiters", "i")],
+                       "Core.Tuple")
+    root.append(("integration", w.compact(_SCALAR, _shared(nt), True,
+                                          struct.pack("<qdddq", int(n_quad), float(max_point), float(atol),
+                                                      float(rtol), int(maxiters)))))
+    mt = t.named_tuple([("metric_reg", "f"), ("precond_reg_prefactor", "f"), ("precond_reg", "f")], "Tuple")
+    root.append(("manifold", w.compact(_SCALAR, _shared(mt), True, struct.pack("<ddd", *map(float, fields_["manifold"])))))
+    types = sorted(w.types)   # _types/00000001.. in the order the types were placed
+    root.append(("_types", w.group([(f"{i + 1:08d}", a) for i, a in enumerate(types)])))
+    root_addr = w.group(root)
+    w.flush_heap()
+    eof = _BASE + len(w.buf)
+    sb = b"\x89HDF\r\n\x1a\n\x02\x08\x08\x00" + struct.pack("<QQQQ", _BASE, _UNDEF, eof, root_addr)
+    w.at(0, sb + struct.pack("<I", lookup3(sb)))
+    content = PREAMBLE + bytes(_BASE - len(PREAMBLE)) + bytes(w.buf)
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    fd, tmp = tempfile.mkstemp(dir=path.parent, suffix=".jld2.tmp")
+    try:
+        with os.fdopen(fd, "wb") as f:
+            f.write(content)
+        os.replace(tmp, path)
+    finally:
+        if os.path.exists(tmp):
+            os.remove(tmp)
+    return path
