@@ -246,8 +246,6 @@ def test_d16_eigsolve_matches_golden_level(rg):
     assert abs(v["refined_energy"] - e["four_energies"][1]) <= e["ritz_tolerance"]
 
 
-@pytest.mark.skipif(os.environ.get("RCMPS_SLOW_EXCITATION") != "1",
-                    reason="several minutes on one MI355X: set RCMPS_SLOW_EXCITATION=1")
 @pytest.mark.parametrize("name", ["D32_odd_p0.05", "D32_even_p0"])
 def test_d32_eigsolve_matches_golden_level(rg, name):
     """The optimized D = 32 Z2 ground state (g = 2.7) at the GPU-comparison settings."""
@@ -314,3 +312,41 @@ def test_repeated_actions_release_their_workspaces(rg):
     free1 = torch.cuda.mem_get_info()[0]
     note(f"  free memory change {(free1 - free0) / 2**20:.1f} MiB")
     assert free0 - free1 < 64 * 2**20
+
+
+def test_nine_momentum_dispersion_lockstep(rg):
+    """The Z2 dispersion workload: RandomStateZ2(32) (scaling 0.2, seed 42), g = 2, odd sector, nev = 2
+    at p = 0, 0.25, ..., 2 with the nine Davidson searches in lockstep (one device call applies one
+    action per momentum) at the reference's default solver settings.  Properties: the sector has
+    dimension 512; at every momentum the two Ritz levels are ordered; every accepted level has its four
+    validation energies within the acceptance spread; most momenta accept both levels; the lowest level
+    rises from p = 0 to p = 2; and the p = 0 search equals a standalone excitation_eigsolve at p = 0
+    (same start, same operator count, energies to 1e-10)."""
+    note("case: dispersion RandomStateZ2(32) g=2 odd nev=2 p=0:0.25:2 lockstep")
+    from rcmps_gpu import IntegratorParameters, iterative_dispersion
+    r, k = rg.random_state_z2_arrays(32, scaling=0.2, seed=42)
+    s = rg.State(r, k)
+    s.integrator_param = IntegratorParameters(n_quad=128, max_point=20.0, atol=1e-11, rtol=1e-10)
+    s.model_param.name = "phi4"
+    s.model_param.coupling = 2.0
+    momenta = [0.25 * i for i in range(9)]
+    rs = iterative_dispersion(s, momenta, nev=2, sector="odd", lockstep=True, strict=False)
+    lowest = []
+    accepted_both = 0
+    for p, res in zip(momenta, rs):
+        note(f"  p={p}: energies {res['energies']} numops {res['numops']} accepted {res['accepted']} "
+             f"stop {res['stop_reason']}")
+        assert res["dimension"] == 512 and len(res["energies"]) == 2
+        assert res["energies"][0] <= res["energies"][1] + 1e-12
+        for v in res["validation"]:
+            if v["accepted"]:
+                assert v["energy_spread"] <= res["validation_tol"]
+        accepted_both += bool(res["accepted"])
+        lowest.append(res["energies"][0])
+    assert accepted_both >= 7
+    assert lowest[-1] > lowest[0]
+    alone = rg.excitation_eigsolve(s, 0.0, nev=2, sector="odd")
+    note(f"  standalone p=0: energies {alone['energies']} numops {alone['numops']}")
+    assert alone["numops"] == rs[0]["numops"]
+    np.testing.assert_allclose(alone["energies"], rs[0]["energies"], rtol=0, atol=1e-10)
+    s.close()
