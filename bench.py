@@ -138,6 +138,55 @@ def batch_main(args):
     return 0
 
 
+def action_main(args):
+    """One step = one Hamiltonian action F(W) on the reference's optimized Z2 checkpoint (D = 16, g = 2
+    or D = 32, g = 2.7; tests/golden) at the settings of its GPU comparison runs: window 30, n_quad 512,
+    1e-13/1e-12, 256 theta nodes, p = 0.1, a seeded odd tangent.  Reports the kinetic and :phi^4: parts
+    apart as well."""
+    import numpy as np
+    import torch
+
+    import rcmps_gpu
+    from rcmps_gpu import IntegratorParameters
+    from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
+    from rcmps_gpu.jld2 import read_jld2
+    D = args.action
+    name = {16: "phi4_g2_D16_z2_state", 32: "phi4_g2.7_D32_z2_state"}.get(D)
+    if name is None:
+        print("bench.py: --action takes 16 or 32 (the reference's optimized checkpoints)", file=sys.stderr)
+        return 2
+    rec = read_jld2(ROOT / "tests" / "golden" / f"{name}.jld2")
+    s = rcmps_gpu.State(rec.R, rec.K, steady_tol=1e-12)
+    s.integrator_param = IntegratorParameters(n_quad=512, max_point=30.0, atol=1e-13, rtol=1e-12)
+    ctx = ExcitationContext(s, 0.1)
+    model = ExcitationModel("phi4", rec.coupling)
+    rng = np.random.default_rng(7)
+    w = rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D))
+    w[:D // 2, :D // 2] = 0
+    w[D // 2:, D // 2:] = 0
+    times = {}
+    for parts, label in ((0, "action"), (2, "phi4_part"), (1, "kinetic_part")):
+        for _ in range(args.warmup):
+            action_batch(ctx, [w], model, 256, parts=parts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, st = action_batch(ctx, [w], model, 256, parts=parts, return_stats=True)
+        torch.cuda.synchronize()
+        times[label] = ((time.perf_counter() - t0) / args.steps, st)
+    sec, st = times["action"]
+    out = {"metric": f"excitation Hamiltonian action wall time, D={D}", "value": sec, "unit": "s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": f"reference checkpoint {name}",
+           "config": {"workload": f"{name}, p=0.1, seeded odd tangent, window 30, n_quad 512, 1e-13/1e-12, "
+                                  "256 theta nodes"},
+           "phi4_part_s": times["phi4_part"][0], "kinetic_part_s": times["kinetic_part"][0],
+           "ode_ms": st["trajectory_ms"], "kinetic_actions": st["kinetic_actions"], "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+    s.close()
+    return 0
+
+
 def excitation_main(args):
     """One step = the whole lockstep dispersion solve (preconditioners, Davidson, validation)."""
     import numpy as np
@@ -206,13 +255,15 @@ def main(argv=None):
     ap.add_argument("--excitation", action="store_true",
                     help="time the lockstep excitation dispersion (9 momenta, nev=2) instead")
     ap.add_argument("--exc-D", type=int, default=32)
+    ap.add_argument("--action", type=int, default=0, metavar="D",
+                    help="time one excitation action on the reference's optimized D=16 or D=32 checkpoint")
     ap.add_argument("--exc-chunk", type=int, default=128, help="tangents per coarse kinetic action call")
     args = ap.parse_args(argv)
     # the dispersion solve is minutes long: one timed step and no warmup unless asked for
     if args.steps is None:
-        args.steps = 1 if args.excitation else 5
+        args.steps = 1 if args.excitation else 2 if args.action else 5
     if args.warmup is None:
-        args.warmup = 0 if args.excitation else 2
+        args.warmup = 0 if args.excitation else 1 if args.action else 2
     if args.steps < 1 or args.warmup < 0:
         print("bench.py: --steps must be at least 1 and --warmup at least 0", file=sys.stderr)
         return 2
@@ -224,6 +275,8 @@ def main(argv=None):
         return batch_main(args)
     if args.excitation:
         return excitation_main(args)
+    if args.action:
+        return action_main(args)
     import numpy as np
     import torch
 

@@ -40,7 +40,7 @@ extern "C" {
 #define RCMPS_ERR_STEP_UNDERFLOW 2   /* ODE step below 10 ulp */
 #define RCMPS_ERR_NONFINITE 3        /* nonfinite error estimate or result */
 #define RCMPS_ERR_MISSING_SAMPLES 5  /* trajectory ended before every quadrature node */
-#define RCMPS_ERR_WAIT_LIMIT 7       /* a workgroup waited past the spin limit (kernel aborted) */
+#define RCMPS_ERR_WAIT_LIMIT 7       /* a workgroup waited past its time budget (kernel aborted) */
 #define RCMPS_ERR_SOLVER 10          /* Krylov or factorisation failure (SolverFailure) */
 #define RCMPS_ERR_ARGUMENT 11        /* invalid argument (ArgumentError) */
 #define RCMPS_ERR_DEVICE 12          /* HIP runtime error */
@@ -57,6 +57,13 @@ extern "C" {
    the angular resolvent with matrix-free Arnoldi (the context's rcmps_kinetic_options); only the
    :phi^4: trajectories run. */
 #define RCMPS_FLAG_KINETIC_KRYLOV 4
+/* fill rcmps_stats.phase_us (the trajectory kernel's per-phase timers); without it the kernel reads
+   no clock and phase_us stays zero */
+#define RCMPS_FLAG_PHASE_TIMERS 8
+/* ignore a state's Z2 structure: every product over every k-tile (the same bits as the default,
+   which skips the k-tiles of the parity blocks that are exactly zero when R is odd and K, rho0 even
+   under the split of the basis into halves of whole 16 x 16 tiles, D = 32, 64) */
+#define RCMPS_FLAG_DENSE 16
 
 typedef struct rcmps_context rcmps_context;
 typedef struct rcmps_state rcmps_state;

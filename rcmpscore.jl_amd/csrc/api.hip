@@ -212,6 +212,9 @@ struct rcmps_context {
 
 struct rcmps_state {
     int D = 0, Dp = 0;
+    // Z2 symmetry under diag(1, .., 1, -1, .., -1) (D/2 each): R odd, K and rho0 even, exactly.  When
+    // the parity blocks are whole 16 x 16 tiles, z2_tiles = D / 32 (tiles per parity block), else 0.
+    int z2_tiles = 0;
     DevBuf R, K, Q, Qh, Rh, C, Ch, rho, V, Vinv, lam, ident;
     double shift = 0.0;
     std::vector<cd> rho_host;
@@ -662,6 +665,34 @@ bool checked_density(int D, const Mat& q, const Mat& r, Mat& rho, double tol) {
     return mn >= -10 * tol;
 }
 
+// Tiles per parity block when R is exactly odd and K, rho0 exactly even under the split of the basis
+// into halves, and the halves are whole tiles (D a multiple of 32); 0 otherwise.
+int z2_tiles_of(int D, const Mat& r, const Mat& k, const Mat& rho) {
+    if (D % 32 != 0) return 0;
+    const int d = D / 2;
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            const bool same = (i < d) == (j < d);
+            const size_t e = (size_t)i * D + j;
+            if (same && r[e] != cd(0, 0)) return 0;
+            if (!same && (k[e] != cd(0, 0) || rho[e] != cd(0, 0))) return 0;
+        }
+    return d / 16;
+}
+
+// k-tile mask of a tile product A(b, :) B(:, c) whose operands have parities pa, pb (1: odd) under
+// parity blocks of `zt` tiles (0: no structure, every k-tile).  The product is nonzero only when the
+// output tile's parity pa ^ pb matches its blocks, and then only the k-tiles of block blk(b) ^ pa count.
+int z2_kmask(int NB, int zt, int b, int c, int pa, int pb) {
+    if (zt == 0) return 15;
+    const int bb = b / zt, bc = c / zt;
+    if ((bb ^ bc) != (pa ^ pb)) return 0;
+    int m = 0;
+    for (int t = 0; t < NB; ++t)
+        if (t / zt == (bb ^ pa)) m |= 1 << t;
+    return m;
+}
+
 int guard(const std::function<void()>& f) {
     try {
         f();
@@ -846,6 +877,7 @@ int rcmps_state_create(rcmps_context* ctx, int D, const double* R, const double*
         if (!checked_density(D, qh, rh, rho_h, steady_tol)) throw Fail{RCMPS_ERR_SOLVER};
         upload(ctx, D, Dp, rho_h, st->rho.d());
         st->rho_host = rho_h;
+        st->z2_tiles = z2_tiles_of(D, rh, kh, rho_h);
         ck(hipStreamSynchronize(ctx->stream));
     });
     if (rc) {
@@ -989,9 +1021,13 @@ void ensure_quadrature(rcmps_context* ctx, const rcmps_integrator* ip) {
 
 // Fixed insertion sources, the six systems and their tasks for one state (src/observables.jl:7-29:
 // kinetic edges sqrt2 with op R or [Q,R], :phi^4: edges k -> k+1 with coefficient k).
-void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool adjoint, bool quartic_only, int npts,
-                 std::vector<TrajSys>& sys, std::vector<TrajTask>& tasks) {
+void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool adjoint, bool quartic_only, bool z2,
+                 int npts, std::vector<TrajSys>& sys, std::vector<TrajTask>& tasks) {
     const int D = st->D, Dp = st->Dp, NB = Dp / 16, NT = NB * NB;
+    // parity of every live channel's stage input for a Z2 state (rho0 and I even; R, R^+, [Q,R] odd;
+    // the insertions and the fixed sources flip it once per operator factor)
+    static const int kParity[RC_NSYS][RC_MAX_LIVE] = {{1}, {0, 1}, {1}, {0, 1}, {1, 0, 1}, {0, 1, 0, 1}};
+    const int zt = z2 ? st->z2_tiles : 0;
     Dev dv{ctx, Dp, D};
     const size_t mb = (size_t)Dp * Dp * 16;
     const size_t mcount = (size_t)Dp * Dp * 2;
@@ -1058,6 +1094,13 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
                     TrajTask t{};
                     t.sys = sys_base + s; t.ch = ch; t.b = b; t.c = c; t.adj = adj;
                     t.edge_src = -1;
+                    {
+                        const int px = kParity[s][ch];
+                        const int m1 = adj ? z2_kmask(NB, zt, b, c, px, 1) : z2_kmask(NB, zt, b, c, 1, px);
+                        const int m2a = z2_kmask(NB, zt, b, c, 0, px), m2b = z2_kmask(NB, zt, b, c, px, 0);
+                        const int m3 = adj ? z2_kmask(NB, zt, b, c, 1, px ^ 1) : z2_kmask(NB, zt, b, c, px ^ 1, 1);
+                        t.kmask = m1 | (m2a << 4) | (m2b << 8) | (m3 << 12);
+                    }
                     if (term < 2) {
                         const double* O = term == 0 ? st->R.d() : st->C.d();
                         if (!adj) {
@@ -1090,6 +1133,7 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
                     t.edge_src = -1;
                     t.has_E = 1;
                     t.e_only = 1;
+                    t.kmask = z2_kmask(NB, zt, b, c, kParity[3][1], 1) << 16;   // E = X C of channel 1
                     tasks.push_back(t);
                 }
         sys.push_back(Sy);
@@ -1098,8 +1142,8 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
 
 // Integrate every system of the given evaluations (all of one padded size Dp) in one persistent
 // launch; states are packed into rounds of at most the resident workgroup count.
-void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, const rcmps_integrator* ip,
-               const std::vector<int>& idx, std::vector<EvalReq>& reqs) {
+void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, bool timers, bool z2,
+               const rcmps_integrator* ip, const std::vector<int>& idx, std::vector<EvalReq>& reqs) {
     const int NB = Dp / 16, npts = 2 * ip->n_quad + 1;
     std::vector<TrajSys> sys;
     std::vector<TrajTask> tasks;
@@ -1107,7 +1151,7 @@ void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, const 
     std::vector<int> state_first, state_ntasks;
     for (size_t j = 0; j < idx.size(); ++j) {
         const int first = (int)tasks.size();
-        build_state(ctx, *ctx->tb[j], reqs[idx[j]].st, reqs[idx[j]].adjoint(), quartic_only, npts, sys, tasks);
+        build_state(ctx, *ctx->tb[j], reqs[idx[j]].st, reqs[idx[j]].adjoint(), quartic_only, z2, npts, sys, tasks);
         state_first.push_back(first);
         state_ntasks.push_back((int)tasks.size() - first);
     }
@@ -1130,12 +1174,14 @@ void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, const 
     P.Dp = Dp;
     P.npoints = npts;
     P.nrounds = (int)rounds.size();
+    P.timers = timers ? 1 : 0;
     P.a = std::sqrt(ip->max_point);
     P.atol = ip->atol;
     P.rtol = ip->rtol;
     P.h0 = ip->initial_step > 0 ? ip->initial_step : 0.01;
     P.max_steps = ip->maxiters;
-    P.spin_limit = 20000000LL;
+    // a single wait longer than a minute means a partner is not running: abort with WAIT_LIMIT
+    P.wait_ticks = (long long)(60.0 * ctx->clock_mhz * 1e6);
     P.points = ctx->traj_points.d();
     P.abort_flag = static_cast<uint32_t*>(ctx->abort_word.p);
     ck(ctx->traj_tasks.ensure(tasks.size() * sizeof(TrajTask)));
@@ -1932,7 +1978,8 @@ int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcm
             for (size_t i = 0; i < reqs.size(); ++i)
                 if (reqs[i].st->Dp == Dp) idx.push_back((int)i);
             if (idx.empty()) continue;
-            run_class(ctx, Dp, (flags & RCMPS_FLAG_SPLIT) != 0, krylov, ip, idx, reqs);
+            run_class(ctx, Dp, (flags & RCMPS_FLAG_SPLIT) != 0, krylov, (flags & RCMPS_FLAG_PHASE_TIMERS) != 0,
+                      (flags & RCMPS_FLAG_DENSE) == 0, ip, idx, reqs);
             if (!krylov && finish_concurrently(ctx, ip, metric_reg, flags, idx, reqs)) continue;
             for (size_t j = 0; j < idx.size(); ++j) {
                 EvalReq& r = reqs[idx[j]];

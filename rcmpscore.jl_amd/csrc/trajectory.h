@@ -8,7 +8,7 @@
 #define RC_REC 16           // doubles per workgroup record: e5, e3, 12 trace partials, spare
 
 // status codes (shared with the C-ABI): 1-5 as the DOP853 controller of the reference GPU
-// package (device_control_backend.py:23-27), 7 = a workgroup waited past the spin limit.
+// package (device_control_backend.py:23-27), 7 = a workgroup waited past its time budget.
 #define RC_ST_OK 0
 #define RC_ST_MAX_STEPS 1
 #define RC_ST_STEP_UNDERFLOW 2
@@ -25,6 +25,12 @@ struct TrajTask {
     int trace_kind;     // 0 none; 1: Re sum X conj(O) over the tile; 2: Re diag of the level-1 tile
     int e_only;         // 1: only form and publish channel ch's E tile each stage (no stage input,
                         // no samples, an empty error record)
+    // k-tiles that contribute to each tile product, four bits per product: level 1 [0, 4), the two
+    // level-2a products [4, 8) and [8, 12), level 2b [12, 16), the E product [16, 20).  All ones
+    // for a general state; for a Z2-symmetric state only the k-tiles of the one parity block that
+    // can be nonzero (none when the product is structurally zero): the skipped k-steps multiply
+    // exact zeros, so the results are the same bits.
+    int kmask;
     double edge_coef;
     double trace_coef;
     const double* fixed;     // Dp x Dp complex (row-major, interleaved) or null
@@ -51,9 +57,9 @@ struct TrajSys {
 // are gang-scheduled in rounds: workgroup w runs task schedule[r * nblocks + w] (or idles, -1)
 // in round r, and all tasks of a system share a round, so every wait is on a running partner.
 struct TrajParams {
-    int Dp, npoints, nrounds, pad0;
+    int Dp, npoints, nrounds, timers;   // timers: fill the per-phase timers of workgroup 0
     double a, atol, rtol, h0;
-    long long max_steps, spin_limit;
+    long long max_steps, wait_ticks;   // wait_ticks: time budget of one wait (constant-rate clock ticks)
     const double* points;
     const double* cheb;
     const TrajTask* tasks;

@@ -19,7 +19,7 @@
 // a grid barrier: each waits on per-tile event flags of the tiles it reads; data written for
 // other workgroups uses agent-coherent stores and loads.  The step-end error norm is a
 // deterministic all-gather of per-workgroup records, so every workgroup of a system takes the
-// same controller decision.  Every wait is bounded by a spin limit and watches an abort word.
+// same controller decision.  Every wait is bounded by a time budget and watches an abort word.
 #include "device_common.h"
 #include "split.h"
 #include "trajectory.h"
@@ -42,25 +42,32 @@ struct Lay {
 
 struct Ctl {
     uint32_t* abort_flag;
-    long long spin_limit;
+    long long wait_ticks;   // longest single wait, in ticks of the constant-rate clock (s_memrealtime)
 };
 
 // Wave 0 polls up to 64 flags (lane l: flags[l] >= target); all waves then meet.  Returns false
-// when the abort word is set or the spin limit is passed (then the abort word is set).
+// when the abort word is set or the wait outlasts its time budget (then the abort word is set).
+// The budget is wall-clock time, not a poll count: a workgroup that starts a later round of a
+// batched launch early may wait a whole round for partners that are still running.
 __device__ bool wait_flags(const Ctl& ctl, uint32_t* const* addr_lane, int n, uint32_t target,
                            int* s_abort) {
     const int tid = threadIdx.x, lane = tid & 63;
     if (tid < 64) {
         uint32_t* a = lane < n ? addr_lane[lane] : nullptr;
-        long long spins = 0;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
         int bad = 0;
-        while (true) {
+        for (uint32_t polls = 1;; ++polls) {
             // the flag and the abort word are loaded together: one round trip per poll
             const uint32_t fv = a ? ld_flag(a) : 0u;
             const uint32_t ab = ld_flag(ctl.abort_flag);
             const bool ok = a ? (fv >= target) : true;
             if (__all(ok)) break;
-            if (++spins > ctl.spin_limit || ab) { bad = 1; break; }
+            if (ab) { bad = 1; break; }
+            // the clock is read every 64th poll only
+            if ((polls & 63) == 0 && (long long)(__builtin_amdgcn_s_memrealtime() - t0) > ctl.wait_ticks) {
+                bad = 1;
+                break;
+            }
             __builtin_amdgcn_s_sleep(1);
         }
         if (lane == 0) {
@@ -92,11 +99,14 @@ __device__ __forceinline__ cplx fragB(const cplx* N, int ks, int lane) {
     return N[(4 * ks + (lane >> 4)) * 16 + (lane & 15)];
 }
 
-// acc += A(row block) * B(col block) over this wave's share of the k-steps
+// acc += A(row block) * B(col block) over this wave's share of the k-steps, k-tiles in `kmask`
+// only (uniform; bit t: k-tile t).  Each wave runs one k-step of every k-tile.
 template <int NB>
-__device__ __forceinline__ void tile_product(cacc& acc, const cplx* A, const cplx* B, int wave, int lane) {
+__device__ __forceinline__ void tile_product(cacc& acc, const cplx* A, const cplx* B, int wave, int lane,
+                                             int kmask) {
 #pragma unroll
-    for (int ks = wave; ks < 4 * NB; ks += 4) cmfma(acc, fragA<NB>(A, ks, lane), fragB(B, ks, lane));
+    for (int ks = wave; ks < 4 * NB; ks += 4)
+        if ((kmask >> (ks >> 2)) & 1) cmfma(acc, fragA<NB>(A, ks, lane), fragB(B, ks, lane));
 }
 
 // Sum the four waves' partial tiles; thread t receives element (t/16, t%16).
@@ -467,10 +477,13 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
     const TrajTask tk = load_scalar<TrajTask>(p.tasks, task);
     const TrajSys S = load_scalar<TrajSys>(p.sys, tk.sys);
     const int b = tk.b, c = tk.c, ch = tk.ch, adj = tk.adj;
+    // k-tile masks of the tile products (uniform)
+    const int m1 = tk.kmask & 15, m2a = (tk.kmask >> 4) & 15, m2b = (tk.kmask >> 8) & 15,
+              m3 = (tk.kmask >> 12) & 15, mE = (tk.kmask >> 16) & 15;
     const int row = 16 * b + ei, col = 16 * c + ej;
     const bool e_only = tk.e_only != 0;                  // uniform over the workgroup
     const bool real_el = !e_only && row < S.D && col < S.D;
-    const Ctl ctl{p.abort_flag, p.spin_limit};
+    const Ctl ctl{p.abort_flag, p.wait_ticks};
     const uint32_t mat_bytes = (uint32_t)(Dp * Dp * 16);
     const uint32_t stage_stride = (uint32_t)S.nlive * mat_bytes;
     const uint32_t ybytes = RC_NSTAGE_BUF * stage_stride;
@@ -529,15 +542,20 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
     // phase timers (constant-rate clock ticks): wait Y, load X, level 1 + publish P, level 2a,
     // wait P, load P/insertions, level 2b + epilogue, integrator bookkeeping, step end (error
     // norm, record, all-gather), total, publish Y, stage entry
+    // The timers run only when the caller asks for them: each reading of the clock waits for the
+    // scalar memory counter, which the untimed kernel does not pay.
+    const bool timed = uni(p.timers) != 0;
     uint64_t tacc[RC_NTIMERS];
 #pragma unroll
     for (int q = 0; q < RC_NTIMERS; ++q) tacc[q] = 0;
-    uint64_t tmark = __builtin_amdgcn_s_memrealtime();
+    uint64_t tmark = timed ? __builtin_amdgcn_s_memrealtime() : 0;
     const uint64_t tstart = tmark;
     auto lap = [&](int q) {
-        uint64_t now = __builtin_amdgcn_s_memrealtime();
-        tacc[q] += now - tmark;
-        tmark = now;
+        if (timed) {
+            uint64_t now = __builtin_amdgcn_s_memrealtime();
+            tacc[q] += now - tmark;
+            tmark = now;
+        }
     };
     auto stage = [&](int si, uint32_t ev, double ts, double src, cplx& k_out, double& tr_out) -> bool {
         const double jac = 2.0 * fabs(ts);
@@ -579,12 +597,12 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             if constexpr (SPLIT) {
                 double pe[4] = {0.0, 0.0, 0.0, 0.0};
                 const int s2[4] = {-(sXr[0] + sK4), -(sXr[1] + sK4), -(sXr[2] + sK4), -(sXr[3] + sK4)};
-                rc_split_product(pe, pXr, spa, pK4, spb, s2, lane);
+                if (mE) rc_split_product(pe, pXr, spa, pK4, spb, s2, lane);
                 Ee = reduce_split(pe, redd, wave, lane);
             } else {
                 cacc ae;
                 cacc_zero(ae);
-                tile_product<NB>(ae, Xr, Kc, wave, lane);
+                tile_product<NB>(ae, Xr, Kc, wave, lane, mE);
                 Ee = reduce_tile(ae, red, wave, lane);
             }
             st_coh(rE, (uint32_t)si * mat_bytes + el_off, Ee);
@@ -600,16 +618,16 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
         if constexpr (SPLIT) {
             if (!adj) {
                 const int s2[4] = {-(sK0 + sXc), -(sK0 + sXc), -(sK0 + sXc), -(sK0 + sXc)};
-                rc_split_product(part, pK0, spa, pXc, spb, s2, lane);
+                if (m1) rc_split_product(part, pK0, spa, pXc, spb, s2, lane);
             } else {
                 const int s2[4] = {-(sXr[0] + sK0), -(sXr[1] + sK0), -(sXr[2] + sK0), -(sXr[3] + sK0)};
-                rc_split_product(part, pXr, spa, pK0, spb, s2, lane);
+                if (m1) rc_split_product(part, pXr, spa, pK0, spb, s2, lane);
             }
             Pe = reduce_split(part, redd, wave, lane);
         } else {
             cacc_zero(acc);
-            if (!adj) tile_product<NB>(acc, Kl1, Xc, wave, lane);
-            else tile_product<NB>(acc, Xr, Kl1, wave, lane);
+            if (!adj) tile_product<NB>(acc, Kl1, Xc, wave, lane, m1);
+            else tile_product<NB>(acc, Xr, Kl1, wave, lane, m1);
             Pe = reduce_tile(acc, red, wave, lane);
         }
         const uint32_t poff = (uint32_t)si * stage_stride + (uint32_t)ch * mat_bytes;
@@ -627,12 +645,12 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             part[0] = part[1] = part[2] = part[3] = 0.0;
             const int s2a[4] = {-(sK1 + sXc), -(sK1 + sXc), -(sK1 + sXc), -(sK1 + sXc)};
             const int s2b[4] = {-(sXr[0] + sK2), -(sXr[1] + sK2), -(sXr[2] + sK2), -(sXr[3] + sK2)};
-            rc_split_product(part, pK1, spa, pXc, spb, s2a, lane);
-            rc_split_product(part, pXr, spa, pK2, spb, s2b, lane);
+            if (m2a) rc_split_product(part, pK1, spa, pXc, spb, s2a, lane);
+            if (m2b) rc_split_product(part, pXr, spa, pK2, spb, s2b, lane);
         } else {
             cacc_zero(acc);
-            tile_product<NB>(acc, Kq, Xc, wave, lane);
-            tile_product<NB>(acc, Xr, Kx, wave, lane);
+            tile_product<NB>(acc, Kq, Xc, wave, lane, m2a);
+            tile_product<NB>(acc, Xr, Kx, wave, lane, m2b);
         }
         __syncthreads();
         lap(3);
@@ -682,16 +700,16 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
             if (!adj) {
                 const int s2[4] = {-(sgP[4 * (lane >> 4)] + sK3), -(sgP[4 * (lane >> 4) + 1] + sK3),
                                    -(sgP[4 * (lane >> 4) + 2] + sK3), -(sgP[4 * (lane >> 4) + 3] + sK3)};
-                rc_split_product(part, pPb, spa, pK3, spb, s2, lane);
+                if (m3) rc_split_product(part, pPb, spa, pK3, spb, s2, lane);
             } else {
                 const int sc = -(sK3 + sgP[lane & 15]);
                 const int s2[4] = {sc, sc, sc, sc};
-                rc_split_product(part, pK3, spa, pPb, spb, s2, lane);
+                if (m3) rc_split_product(part, pK3, spa, pPb, spb, s2, lane);
             }
             Le = reduce_split(part, redd, wave, lane);
         } else {
-            if (!adj) tile_product<NB>(acc, Pb, Kp, wave, lane);
-            else tile_product<NB>(acc, Kp, Pb, wave, lane);
+            if (!adj) tile_product<NB>(acc, Pb, Kp, wave, lane, m3);
+            else tile_product<NB>(acc, Kp, Pb, wave, lane, m3);
             Le = reduce_tile(acc, red, wave, lane);
         }
         cplx k = cadd(cscale(jac, Le), cscale(src, cadd(ins, fixed_e)));
@@ -935,7 +953,7 @@ __device__ void run_task(const TrajParams& p, int task, unsigned char* smem_raw)
         S.out[RC_OUT_T] = t;
         S.out[RC_OUT_NEXTPOINT] = next_point;
         lap(7);
-        tacc[9] = __builtin_amdgcn_s_memrealtime() - tstart;
+        if (timed) tacc[9] = __builtin_amdgcn_s_memrealtime() - tstart;
 #pragma unroll
         for (int q = 0; q < RC_NTIMERS; ++q) S.out[RC_OUT_TIMERS + q] = (double)tacc[q];
     }

@@ -27,6 +27,8 @@ SYSTEMS = ("kinetic forward", "kinetic adjoint", "kinetic-commutator forward",
 FLAG_NO_BOUNDARY = 1
 FLAG_SPLIT = 2
 FLAG_KINETIC_KRYLOV = 4
+FLAG_PHASE_TIMERS = 8
+FLAG_DENSE = 16
 
 
 class SolverFailure(RuntimeError):

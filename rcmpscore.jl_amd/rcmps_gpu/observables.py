@@ -42,11 +42,13 @@ def _integrator(p: IntegratorParameters, initial_step: float = 0.01) -> _lib.Int
 
 def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: IntegratorParameters | None = None,
                 metric_reg: float | None = None, precision: str = "fp64", kinetic_backend: str = "ode",
-                kinetic_options=None, krylov_options=None):
+                kinetic_options=None, krylov_options=None, phase_timers: bool = False, z2_blocks: bool = True):
     """(energy, gradient, raw covector, raw environment, stats dict) on device tensors.
 
     precision="split" runs the trajectory products on the INT8 MFMA over exact 8-bit slices of
-    the FP64 operands; everything else stays FP64."""
+    the FP64 operands; everything else stays FP64.  phase_timers=True fills stats["phase_us"]
+    (the trajectory kernel's per-phase timers, which otherwise read no clock).  z2_blocks=False makes
+    the trajectory products ignore a Z2 state's zero parity blocks (the same bits, every k-tile)."""
     lib = _lib.load()
     ctx = _lib.context(s.device)
     ip = _integrator(integrator or s.integrator_param)
@@ -60,7 +62,8 @@ def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: Integr
     env = torch.empty_like(G)
     e = ctypes.c_double()
     stats = _lib.Stats()
-    flags = _flags(boundary, precision, kinetic_backend)
+    flags = (_flags(boundary, precision, kinetic_backend) | (_lib.FLAG_PHASE_TIMERS if phase_timers else 0)
+             | (0 if z2_blocks else _lib.FLAG_DENSE))
     if kinetic_backend == "krylov":
         from .kinetic import apply_options
         apply_options(s.device, kinetic_options, krylov_options)
@@ -72,7 +75,7 @@ def fg_phi4_raw(s: State, g: float, *, boundary: bool = True, integrator: Integr
 
 def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str = "ode", parallel=None,
             boundary: bool = True, return_stats: bool = False, precision: str = "fp64", kinetic_options=None,
-            krylov_options=None):
+            krylov_options=None, phase_timers: bool = False, z2_blocks: bool = True):
     """fg_phi4(s, g) -> (energy, gradient) with dE[W] = metric(s, gradient, W) for horizontal W.
 
     gradient=False returns the energy only.  `parallel` is accepted for signature compatibility:
@@ -80,7 +83,8 @@ def fg_phi4(s: State, g: float, *, gradient: bool = True, kinetic_backend: str =
     if parallel not in PARALLEL:
         raise ValueError("parallel must be :serial, :terms, :trajectories, or :both")
     energy, G, _, _, stats = fg_phi4_raw(s, g, boundary=boundary, precision=precision, kinetic_backend=kinetic_backend,
-                                         kinetic_options=kinetic_options, krylov_options=krylov_options)
+                                         kinetic_options=kinetic_options, krylov_options=krylov_options,
+                                         phase_timers=phase_timers, z2_blocks=z2_blocks)
     if not gradient:
         return (energy, stats) if return_stats else energy
     grad = G.cpu().numpy()

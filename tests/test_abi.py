@@ -50,8 +50,9 @@ def test_package_import_does_not_touch_the_device():
 def test_flag_values_match_header():
     import rcmps_gpu._lib as L
     flags = dict(re.findall(r"#define RCMPS_FLAG_([A-Z_]+)\s+(\d+)", HEADER.read_text()))
-    assert flags == {"NO_BOUNDARY": "1", "SPLIT": "2", "KINETIC_KRYLOV": "4"}
-    assert (L.FLAG_NO_BOUNDARY, L.FLAG_SPLIT, L.FLAG_KINETIC_KRYLOV) == (1, 2, 4)
+    assert flags == {"NO_BOUNDARY": "1", "SPLIT": "2", "KINETIC_KRYLOV": "4", "PHASE_TIMERS": "8", "DENSE": "16"}
+    assert (L.FLAG_NO_BOUNDARY, L.FLAG_SPLIT, L.FLAG_KINETIC_KRYLOV, L.FLAG_PHASE_TIMERS, L.FLAG_DENSE) == \
+        (1, 2, 4, 8, 16)
 
 
 def test_struct_sizes_match_header():
