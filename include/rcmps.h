@@ -72,6 +72,9 @@ typedef struct rcmps_excitation rcmps_excitation;
 /* parts of rcmps_excitation_action (0 = both) */
 #define RCMPS_EXC_KINETIC 1       /* the kinetic terms on the theta grid */
 #define RCMPS_EXC_POTENTIAL 2     /* the :phi^4: term by the action ODEs */
+#define RCMPS_EXC_DENSE 4         /* with either: ignore a Z2 state's parity blocks in the action ODE
+                                     (the same bits as the default, which skips the exactly zero ones
+                                     when every tangent has one parity and D = 32, 64) */
 
 /* IntegratorParameters (src/state.jl:4-10) plus the DOP853 initial step. */
 typedef struct {
@@ -112,6 +115,9 @@ typedef struct {
     int kinetic_basis[4];
     int kinetic_actions;
     double kinetic_residual;
+    /* infinite-line kinetic backend, wall milliseconds: Arnoldi steps, projected shifted solves,
+       forming the node solutions and their true residuals, the rest (node sums, gradient finish) */
+    double kinetic_ms[4];
 } rcmps_stats;
 
 int rcmps_context_create(int device, rcmps_context** out);
@@ -160,6 +166,29 @@ int rcmps_fg_phi4_rk(rcmps_context* ctx, const rcmps_state* st, double g, const 
 /* Set the infinite-line kinetic controls of this context (defaults: 32, 512, 1e-11, 1e-10, 1e-12,
    1e-12, 16, 512).  RCMPS_ERR_ARGUMENT for invalid controls (ArgumentError in the reference). */
 int rcmps_set_kinetic_options(rcmps_context* ctx, const rcmps_kinetic_options* opt);
+
+/* Layout of every D x D matrix passed through this context's entry points (states, gradients,
+   covectors, environments, boundary and excitation operands; not the padded trajectory samples):
+   RCMPS_LAYOUT_ROW_MAJOR (default) or RCMPS_LAYOUT_COLUMN_MAJOR, Julia's own layout and that of
+   the portable checkpoint's .bin files (gpu/src/rcmps_gpu/io.py:374,467), so a Julia caller passes
+   its arrays as they are instead of permutedims copies.  The device works in padded row-major either way; the copies at the
+   boundary transpose.  RCMPS_ERR_ARGUMENT for another value. */
+#define RCMPS_LAYOUT_ROW_MAJOR 0
+#define RCMPS_LAYOUT_COLUMN_MAJOR 1
+int rcmps_set_layout(rcmps_context* ctx, int layout);
+
+/* The no-jump inverse that preconditions the stationary-state, boundary and excitation solves
+   (src/transfer.jl:50-68, NoJumpInverse: a Schur-form trsyl solve) runs in the eigenbasis of Q
+   (elementwise middle step) unless the eigenvectors are ill-conditioned, as for a defective Q
+   (test/runtests.jl:74-87), where it keeps the Schur form and solves the triangular Sylvester
+   system by an anti-diagonal sweep (D <= 64).  States created on `ctx` take the Schur form when
+   the condition estimate |W|_F |W^-1|_F of Q's triangular eigenvectors exceeds `kappa` (default
+   1e6; 0 always, INFINITY never).  The Krylov solutions are exact either way.
+   RCMPS_ERR_ARGUMENT for a negative or NaN kappa. */
+int rcmps_set_nojump_threshold(rcmps_context* ctx, double kappa);
+/* The form a state's no-jump inverse took: *schur = 1 (Schur basis) or 0 (eigenbasis), *kappa =
+   the eigenvector condition estimate (INFINITY when W is singular). */
+int rcmps_state_nojump_form(const rcmps_state* st, int* schur, double* kappa);
 
 /* Tile variant of this context's batched complex products: -1 (default) lets the autotuner pick
    per (padded size, batch class), 0..3 force one (0: one 16x16 tile per four-wave workgroup with

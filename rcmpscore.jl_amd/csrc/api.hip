@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/rcmps.h"
+#include "excitation.h"
 #include "host_la.h"
 #include "linalg.h"
 #include "rcmps_tables.h"
@@ -76,6 +77,7 @@ struct KineticWork {
     DevBuf X, A, LX, T, Yd, zd, wd, wzd, res, hd, tmp;
     int actions = 0;
     double max_res = 0.0;
+    double ms[4] = {0.0, 0.0, 0.0, 0.0};   // Arnoldi, projected solves, solutions + residuals, total
     void release() {
         for (auto& b : basis) { b.V.release(); b = KrylovBasis(); }
         DevBuf* all[] = {&X, &A, &LX, &T, &Yd, &zd, &wd, &wzd, &res, &hd, &tmp};
@@ -208,6 +210,11 @@ struct rcmps_context {
     // contexts (own stream and workspaces) on which the per-state pullback, boundary and metric
     // solves of a batch run concurrently, one host thread each
     std::vector<rcmps_context*> workers;
+    // states created on this context whose eigenvector condition estimate of Q exceeds this apply
+    // the no-jump inverse in the Schur basis instead of the eigenbasis
+    double nojump_kappa = 1e6;
+    // layout of the D x D matrices at the C ABI (RCMPS_LAYOUT_ROW_MAJOR or RCMPS_LAYOUT_COLUMN_MAJOR)
+    int layout = RCMPS_LAYOUT_ROW_MAJOR;
 };
 
 struct rcmps_state {
@@ -216,6 +223,11 @@ struct rcmps_state {
     // the parity blocks are whole 16 x 16 tiles, z2_tiles = D / 32 (tiles per parity block), else 0.
     int z2_tiles = 0;
     DevBuf R, K, Q, Qh, Rh, C, Ch, rho, V, Vinv, lam, ident;
+    // no-jump inverse: eigenbasis (V = eigenvectors of Q, Vinv their inverse, lam the eigenvalues) or,
+    // when nojump_schur, the Schur basis (V = Z, Vinv = Z^+, Tq = T upper triangular, Q = Z T Z^+)
+    int nojump_schur = 0;
+    double nojump_kappa = 0.0;   // condition estimate |W|_F |W^-1|_F of the triangular eigenvectors
+    DevBuf Tq;
     double shift = 0.0;
     std::vector<cd> rho_host;
     double metric_reg_cached = -1.0;
@@ -392,13 +404,25 @@ struct Dev {
 
 double* Dev::small() const { return sm ? sm : ctx->small.d(); }
 
+// Every D x D matrix that crosses the C ABI goes through these two copies: the caller's layout
+// (row-major by default, column-major after rcmps_set_layout) <-> the padded row-major working layout.
 void pad_in(rcmps_context* ctx, int D, int Dp, const double* src, double* dst) {
+    if (ctx->layout == RCMPS_LAYOUT_COLUMN_MAJOR) {
+        hipLaunchKernelGGL(rc_colmajor_in, dim3((Dp * Dp + 255) / 256), dim3(256), 0, ctx->stream, D, Dp, src, dst);
+        ck(hipGetLastError());
+        return;
+    }
     ck(hipMemsetAsync(dst, 0, (size_t)Dp * Dp * 16, ctx->stream));
     ck(hipMemcpy2DAsync(dst, (size_t)Dp * 16, src, (size_t)D * 16, (size_t)D * 16, D, hipMemcpyDeviceToDevice,
                         ctx->stream));
 }
 
 void pad_out(rcmps_context* ctx, int D, int Dp, const double* src, double* dst) {
+    if (ctx->layout == RCMPS_LAYOUT_COLUMN_MAJOR) {
+        hipLaunchKernelGGL(rc_colmajor_out, dim3((D * D + 255) / 256), dim3(256), 0, ctx->stream, D, Dp, src, dst);
+        ck(hipGetLastError());
+        return;
+    }
     ck(hipMemcpy2DAsync(dst, (size_t)D * 16, src, (size_t)Dp * 16, (size_t)D * 16, D, hipMemcpyDeviceToDevice,
                         ctx->stream));
 }
@@ -431,29 +455,48 @@ double fro(const Mat& a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// no-jump inverse in the eigenbasis of Q:  forward  X = P(Y) solves (Q - s/2) X + X (Q - s/2)^+ = -Y,
-// adjoint X = P*(Y) solves (Q - s/2)^+ X + X (Q - s/2) = -Y  (src/transfer.jl:50-68).  Exact when Q
-// is diagonalisable; used only as a right preconditioner, so the Krylov solution is exact regardless.
+// no-jump inverse:  forward  X = P(Y) solves (Q - s/2) X + X (Q - s/2)^+ = -Y,
+// adjoint X = P*(Y) solves (Q - s/2)^+ X + X (Q - s/2) = -Y  (src/transfer.jl:50-68), with the
+// complex shift s/2 -> s/2 + i sigma of the excitation resolvents.  Q = V diag(lam) V^-1 makes the
+// middle step elementwise; for a Q whose eigenvectors are ill-conditioned the state keeps its Schur
+// form Q = Z T Z^+ instead (V = Z, V^-1 = Z^+) and the middle step is the triangular Sylvester
+// recurrence.  Used as a right preconditioner, so the Krylov solutions are exact either way.
 // ------------------------------------------------------------------------------------------
-void nojump_apply(Dev& dv, const rcmps_state* st, const double* Y, double* out, bool adjoint, double* t1,
-                  double* t2) {
+void nojump_middle(Dev& dv, const rcmps_state* st, bool adjoint, double s_re, double s_im, const double* Y,
+                   double* Z) {
+    if (st->nojump_schur) {
+        hipLaunchKernelGGL(rc_sylvester_tri, dim3(1), dim3(256), rc_sylvester_tri_lds(dv.D), dv.ctx->stream, dv.Dp,
+                           dv.D, adjoint ? 1 : 0, s_re, s_im, st->Tq.d(), Y, Z);
+    } else if (s_im == 0.0) {
+        hipLaunchKernelGGL(rc_sylvester_diag, dv.eb(), dim3(256), 0, dv.ctx->stream, dv.Dp, dv.D, adjoint ? 1 : 0,
+                           s_re, st->lam.d(), Y, Z);
+    } else {
+        hipLaunchKernelGGL(rc_sylvester_diag_c, dv.eb(), dim3(256), 0, dv.ctx->stream, dv.Dp, dv.D, adjoint ? 1 : 0,
+                           s_re, s_im, st->lam.d(), Y, Z);
+    }
+    ck(hipGetLastError());
+}
+
+void nojump_apply_c(Dev& dv, const rcmps_state* st, const double* Y, double* out, bool adjoint, double sigma,
+                    double* t1, double* t2) {
     if (!adjoint) {
         dv.gemm(0, st->Vinv.d(), 0, Y, t1);
         dv.gemm(0, t1, 1, st->Vinv.d(), t2);
-        hipLaunchKernelGGL(rc_sylvester_diag, dv.eb(), dim3(256), 0, dv.ctx->stream, dv.Dp, dv.D, 0, st->shift,
-                           st->lam.d(), t2, t1);
-        ck(hipGetLastError());
+        nojump_middle(dv, st, false, st->shift, sigma, t2, t1);
         dv.gemm(0, st->V.d(), 0, t1, t2);
         dv.gemm(0, t2, 1, st->V.d(), out);
     } else {
         dv.gemm(1, st->V.d(), 0, Y, t1);
         dv.gemm(0, t1, 0, st->V.d(), t2);
-        hipLaunchKernelGGL(rc_sylvester_diag, dv.eb(), dim3(256), 0, dv.ctx->stream, dv.Dp, dv.D, 1, st->shift,
-                           st->lam.d(), t2, t1);
-        ck(hipGetLastError());
+        nojump_middle(dv, st, true, st->shift, -sigma, t2, t1);
         dv.gemm(1, st->Vinv.d(), 0, t1, t2);
         dv.gemm(0, t2, 0, st->Vinv.d(), out);
     }
+}
+
+void nojump_apply(Dev& dv, const rcmps_state* st, const double* Y, double* out, bool adjoint, double* t1,
+                  double* t2) {
+    nojump_apply_c(dv, st, Y, out, adjoint, 0.0, t1, t2);
 }
 
 // L(X) = Q X + X Q^+ + R X R^+ ; L*(X) = Q^+ X + X Q + R^+ X R
@@ -607,10 +650,24 @@ void state_spectral(rcmps_context* ctx, rcmps_state* st) {
     if (!rc::schur(D, T, Z)) throw Fail{RCMPS_ERR_SOLVER};
     Mat W, Winv;
     rc::triangular_eigenvectors(D, T, W);
-    if (!rc::upper_inverse(D, W, Winv)) throw Fail{RCMPS_ERR_SOLVER};
+    const bool inv_ok = rc::upper_inverse(D, W, Winv);
+    // condition estimate of the eigenvectors (Z is unitary): |W|_F |W^-1|_F >= cond_2(W)
+    st->nojump_kappa = inv_ok ? fro(W) * fro(Winv) : INFINITY;
+    if (!std::isfinite(st->nojump_kappa)) st->nojump_kappa = INFINITY;
+    st->nojump_schur = (st->nojump_kappa > ctx->nojump_kappa && D <= RC_SYLVESTER_TRI_MAX_D) ? 1 : 0;
+    if (!st->nojump_schur && !inv_ok) throw Fail{RCMPS_ERR_SOLVER};
     Mat V, Vinv;
-    rc::matmul(D, Z, W, V);
-    rc::matmul(D, Winv, rc::adjoint(D, Z), Vinv);
+    if (st->nojump_schur) {
+        V = Z;
+        Vinv = rc::adjoint(D, Z);
+        ck(st->Tq.ensure((size_t)Dp * Dp * 16));
+        upload(ctx, D, Dp, T, st->Tq.d());
+        ck(hipFuncSetAttribute((const void*)rc_sylvester_tri, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)rc_sylvester_tri_lds(RC_SYLVESTER_TRI_MAX_D)));
+    } else {
+        rc::matmul(D, Z, W, V);
+        rc::matmul(D, Winv, rc::adjoint(D, Z), Vinv);
+    }
     double nq = rc::opnorm2(D, q), nr = rc::opnorm2(D, r);
     double scale = std::max(1.0, std::max(nq, nr * nr));
     double maxre = -INFINITY;
@@ -780,7 +837,7 @@ void rcmps_context_destroy(rcmps_context* ctx) {
 void rcmps_state_destroy(rcmps_state* st) {
     if (!st) return;
     DevBuf* all[] = {&st->R, &st->K, &st->Q, &st->Qh, &st->Rh, &st->C, &st->Ch, &st->rho,
-                     &st->V, &st->Vinv, &st->lam, &st->ident, &st->Minv};
+                     &st->V, &st->Vinv, &st->lam, &st->ident, &st->Minv, &st->Tq};
     for (auto* b : all) b->release();
     delete st;
 }
@@ -1661,20 +1718,34 @@ void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, Krylo
     for (int i = 1; i < nz; ++i)
         if (std::fabs(z[i]) > std::fabs(z[zhard])) zhard = i;
     const std::vector<double> zone(1, z[zhard]);
+    using clock = std::chrono::steady_clock;
+    auto since = [](clock::time_point t0) { return std::chrono::duration<double, std::milli>(clock::now() - t0).count(); };
+    auto extend = [&](int target) {
+        const auto t0 = clock::now();
+        basis_extend(ctx, st, dv, b, adj, target);
+        kw.ms[0] += since(t0);
+    };
+    auto project = [&](const std::vector<double>& zz, int threads, double& e) {
+        const auto t0 = clock::now();
+        const bool r = projected_solutions(b, zz, Y, e, threads);
+        kw.ms[1] += since(t0);
+        return r;
+    };
     while (true) {
-        if (b.k == 0) basis_extend(ctx, st, dv, b, adj, initdim);
+        if (b.k == 0) extend(initdim);
         const int k = b.k;
         const bool terminal = b.exhausted || k >= maxdim;
         double est = 0;
         if (!terminal && nz > 1) {
-            const bool ok1 = projected_solutions(b, zone, Y, est, 1);
+            const bool ok1 = project(zone, 1, est);
             if (!ok1 || est > o.tolerance) {
-                basis_extend(ctx, st, dv, b, adj, std::min(2 * k, maxdim));
+                extend(std::min(2 * k, maxdim));
                 continue;
             }
         }
-        const bool ok = projected_solutions(b, z, Y, est, kProjectedThreads);
+        const bool ok = project(z, kProjectedThreads, est);
         if (ok && (est <= o.tolerance || terminal)) {
+            const auto t0 = clock::now();
             ck(kw.Yd.ensure(Y.size() * 16));
             ck(hipMemcpyAsync(kw.Yd.p, Y.data(), Y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
             hipLaunchKernelGGL(rc_basis_combine, dim3(ceil_div(n, 256), ceil_div(nz, 8)), dim3(256), 0, ctx->stream,
@@ -1686,6 +1757,7 @@ void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, Krylo
             ck(hipGetLastError());
             ck(hipMemcpyAsync(res.data(), kw.res.d(), (size_t)nz * 8, hipMemcpyDeviceToHost, ctx->stream));
             ck(hipStreamSynchronize(ctx->stream));
+            kw.ms[2] += since(t0);
             double worst = 0;
             for (double r2 : res) worst = std::max(worst, std::sqrt(r2) / b.beta);
             const bool fallback = terminal && worst > o.tolerance && worst <= o.fallback_tolerance;
@@ -1696,7 +1768,7 @@ void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, Krylo
             if (b.exhausted) throw Fail{RCMPS_ERR_SOLVER};
         }
         if (terminal) throw Fail{RCMPS_ERR_SOLVER};
-        basis_extend(ctx, st, dv, b, adj, std::min(2 * k, maxdim));
+        extend(std::min(2 * k, maxdim));
     }
 }
 
@@ -1820,6 +1892,8 @@ void kinetic_resolvent_dev(rcmps_context* ctx, const rcmps_state* st, int which,
     for (auto& b : kw.basis) { b.k = 0; b.exhausted = false; b.started = false; b.beta = 0; std::fill(b.H.begin(), b.H.end(), cd(0, 0)); }
     kw.actions = 0;
     kw.max_res = 0;
+    for (double& x : kw.ms) x = 0.0;
+    const auto kin_t0 = std::chrono::steady_clock::now();
     ck(ctx->kry_w.ensure(mb));
     ck(ctx->kry_t1.ensure(mb));
     ck(ctx->small.ensure(64));
@@ -1891,6 +1965,11 @@ void kinetic_resolvent_dev(rcmps_context* ctx, const rcmps_state* st, int which,
                 for (int i = 0; i < 4; ++i) S->kinetic_basis[i] = kw.basis[i].k;
                 S->kinetic_actions = kw.actions;
                 S->kinetic_residual = kw.max_res;
+                ck(hipStreamSynchronize(ctx->stream));
+                const double total =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - kin_t0).count();
+                for (int i = 0; i < 3; ++i) S->kinetic_ms[i] = kw.ms[i];
+                S->kinetic_ms[3] = total - kw.ms[0] - kw.ms[1] - kw.ms[2];
             }
             return;
         }
@@ -1929,6 +2008,7 @@ bool finish_concurrently(rcmps_context* ctx, const rcmps_integrator* ip, double 
     }
     for (int w = 0; w < nw; ++w) {
         ctx->workers[w]->gemm_force = ctx->gemm_force;
+        ctx->workers[w]->layout = ctx->layout;
         ensure_quadrature(ctx->workers[w], ip);
     }
     std::atomic<int> next{0};
@@ -2053,6 +2133,25 @@ int rcmps_fg_phi4_batch(rcmps_context* ctx, int n, const rcmps_state* const* sta
         reqs[i].S = stats ? stats + i : &local[i];
     }
     return fg_phi4_batch_impl(ctx, reqs, ip, metric_reg, flags);
+}
+
+int rcmps_set_layout(rcmps_context* ctx, int layout) {
+    if (!ctx || (layout != RCMPS_LAYOUT_ROW_MAJOR && layout != RCMPS_LAYOUT_COLUMN_MAJOR)) return RCMPS_ERR_ARGUMENT;
+    ctx->layout = layout;
+    return RCMPS_OK;
+}
+
+int rcmps_set_nojump_threshold(rcmps_context* ctx, double kappa) {
+    if (!ctx || !(kappa >= 0)) return RCMPS_ERR_ARGUMENT;
+    ctx->nojump_kappa = kappa;
+    return RCMPS_OK;
+}
+
+int rcmps_state_nojump_form(const rcmps_state* st, int* schur, double* kappa) {
+    if (!st || !schur || !kappa) return RCMPS_ERR_ARGUMENT;
+    *schur = st->nojump_schur;
+    *kappa = st->nojump_kappa;
+    return RCMPS_OK;
 }
 
 int rcmps_gemm_variant(rcmps_context* ctx, int variant) {

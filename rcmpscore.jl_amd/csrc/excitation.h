@@ -31,6 +31,10 @@ struct RcQuarticArgs {
     const double* hp;     // device step size when non-null, else h
     double h, atol, rtol;
     double* partial;
+    // Z2 structure: for R odd and K, rho even with parity blocks of zt whole tiles (0: none), and
+    // every tangent of the launch of parity wpar (0 even, 1 odd), each channel has a definite parity
+    // and the products run only the k-tiles of the one parity block that can be nonzero
+    int zt, wpar;
 };
 
 __global__ void rc_quartic_rhs(RcQuarticArgs g);

@@ -145,26 +145,52 @@ __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, doub
 // ---------------------------------------------------------------------------------------------
 namespace {
 
+// Every fragment of this wave's k-steps (Dp / 16 of them, at most 4) is loaded before the first MFMA,
+// so a product pays one load round trip instead of one per k-step; the MFMAs run in the same k order.
+// Wave w runs k-step w + 4 i of k-tile i.  kmask (uniform) selects the k-tiles: bit i for k-tile i.
 __device__ __forceinline__ void qprod(cacc& acc, const cplx* A, const cplx* B, int Dp, int bi, int bj, int wave,
-                                      int lane, double scale) {
-    for (int ks = wave; ks < Dp / 4; ks += 4) {
-        const int kk = 4 * ks + (lane >> 4);
-        cplx a = A[(16 * bi + (lane & 15)) * Dp + kk];
-        a = cscale(scale, a);
-        const cplx b = B[kk * Dp + 16 * bj + (lane & 15)];
-        cmfma(acc, a, b);
-    }
+                                      int lane, double scale, int kmask = 15) {
+    const int nk = Dp / 16;
+    cplx a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < nk && ((kmask >> i) & 1)) {
+            const int kk = 4 * (wave + 4 * i) + (lane >> 4);
+            a[i] = A[(16 * bi + (lane & 15)) * Dp + kk];
+            b[i] = B[kk * Dp + 16 * bj + (lane & 15)];
+        }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < nk && ((kmask >> i) & 1)) cmfma(acc, cscale(scale, a[i]), b[i]);
 }
 
 // A[tile rows, :] T with T a Dp x 16 block in LDS
 __device__ __forceinline__ void qprod_t(cacc& acc, const cplx* A, const cplx* T, int Dp, int bi, int wave, int lane,
-                                        double scale) {
-    for (int ks = wave; ks < Dp / 4; ks += 4) {
-        const int kk = 4 * ks + (lane >> 4);
-        cplx a = A[(16 * bi + (lane & 15)) * Dp + kk];
-        a = cscale(scale, a);
-        cmfma(acc, a, T[kk * 16 + (lane & 15)]);
-    }
+                                        double scale, int kmask = 15) {
+    const int nk = Dp / 16;
+    cplx a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < nk && ((kmask >> i) & 1)) a[i] = A[(16 * bi + (lane & 15)) * Dp + 4 * (wave + 4 * i) + (lane >> 4)];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < nk && ((kmask >> i) & 1)) {
+            const int kk = 4 * (wave + 4 * i) + (lane >> 4);
+            cmfma(acc, cscale(scale, a[i]), T[kk * 16 + (lane & 15)]);
+        }
+}
+
+// k-tile mask of A(b, :) B(:, c) for operands of parities pa, pb under parity blocks of zt tiles (zt = 0:
+// every k-tile): zero unless the output tile's blocks have parity pa ^ pb, then the k-tiles of block
+// blk(b) ^ pa.  The skipped k-steps multiply exact zeros, so the sums keep their bits.
+__device__ __forceinline__ int zmask(int zt, int nk, int b, int c, int pa, int pb) {
+    if (zt == 0) return 15;
+    const int bb = b / zt, bc = c / zt;
+    if ((bb ^ bc) != (pa ^ pb)) return 0;
+    int m = 0;
+    for (int t = 0; t < nk; ++t)
+        if (t / zt == (bb ^ pa)) m |= 1 << t;
+    return m;
 }
 
 __device__ __forceinline__ cplx qreduce(const cacc& acc, cplx* red, int wave, int lane) {
@@ -177,14 +203,16 @@ __device__ __forceinline__ cplx qreduce(const cacc& acc, cplx* red, int wave, in
     return s;
 }
 
-// T = X1 B1[:, tile] (+ X2 B2[:, tile]) as a Dp x 16 block in LDS
+// T = X1 B1[:, tile] (+ X2 B2[:, tile]) as a Dp x 16 block in LDS; parities p1a, p1b (p2a, p2b) of the
+// operands for the Z2 masks
 __device__ __forceinline__ void qcolumn(cplx* T, const cplx* X1, const cplx* B1, const cplx* X2, const cplx* B2,
-                                        int Dp, int bj, int wave, int lane, cplx* red) {
+                                        int Dp, int bj, int wave, int lane, cplx* red, int zt = 0, int p1a = 0,
+                                        int p1b = 0, int p2a = 0, int p2b = 0) {
     for (int rt = 0; rt < Dp / 16; ++rt) {
         cacc acc;
         cacc_zero(acc);
-        qprod(acc, X1, B1, Dp, rt, bj, wave, lane, 1.0);
-        if (X2) qprod(acc, X2, B2, Dp, rt, bj, wave, lane, 1.0);
+        qprod(acc, X1, B1, Dp, rt, bj, wave, lane, 1.0, zmask(zt, Dp / 16, rt, bj, p1a, p1b));
+        if (X2) qprod(acc, X2, B2, Dp, rt, bj, wave, lane, 1.0, zmask(zt, Dp / 16, rt, bj, p2a, p2b));
         const cplx v = qreduce(acc, red, wave, lane);
         T[(16 * rt + (threadIdx.x >> 4)) * 16 + (threadIdx.x & 15)] = v;
     }
@@ -213,71 +241,77 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
     const cplx* w = reinterpret_cast<const cplx*>(g.Wrep) + (long long)a * 5 * m;
     const double jac = g.coef ? g.coef[0] : g.jac;
     const double jt = g.coef ? g.coef[1] : g.jt;
+    // Z2 parities (g.zt > 0): Q, Q^+ and the shifted ones even, R, R^+ odd, w = W of parity pw,
+    // v = -R^+ W of parity pw ^ 1; channels x_k and a1_k of parity k & 1 (x_0 = rho, a1 of the last
+    // channel = I), y_k and a0_k of parity (k + 1 + pw) & 1 (y_0 = R W, a0 driven by a1 w and a1 v)
+    const int zt = g.zt, nk = Dp / 16, pw = g.wpar;
+    const int px = k & 1, py = (k + 1 + pw) & 1;
+    auto zm = [&](int pa, int pb) { return zmask(zt, nk, bi, bj, pa, pb); };
     cacc acc;
     cacc_zero(acc);
     if (blk == 0) {
         const cplx* x = mat(0, k);
         if (jac != 0.0) {
-            qprod(acc, Q, x, Dp, bi, bj, wave, lane, jac);
-            qprod(acc, x, Qh, Dp, bi, bj, wave, lane, jac);
-            qcolumn(T1, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
-            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
+            qprod(acc, Q, x, Dp, bi, bj, wave, lane, jac, zm(0, px));
+            qprod(acc, x, Qh, Dp, bi, bj, wave, lane, jac, zm(px, 0));
+            qcolumn(T1, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red, zt, px, 1);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac, zm(1, px ^ 1));
         }
         if (k >= 1) {
             const cplx* xp = mat(0, k - 1);
-            qprod(acc, R, xp, Dp, bi, bj, wave, lane, k * jt);
-            qprod(acc, xp, Rh, Dp, bi, bj, wave, lane, k * jt);
+            qprod(acc, R, xp, Dp, bi, bj, wave, lane, k * jt, zm(1, px ^ 1));
+            qprod(acc, xp, Rh, Dp, bi, bj, wave, lane, k * jt, zm(px ^ 1, 1));
         }
     } else if (blk == 1) {
         const cplx* y = mat(1, k);
         const cplx* x = mat(0, k);
         if (jac != 0.0) {
-            qprod(acc, Qs, y, Dp, bi, bj, wave, lane, jac);
-            qprod(acc, y, Qh, Dp, bi, bj, wave, lane, jac);
-            qprod(acc, v, x, Dp, bi, bj, wave, lane, jac);
-            qcolumn(T1, y, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
-            qcolumn(T2, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
-            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
-            qprod_t(acc, w, T2, Dp, bi, wave, lane, jac);
+            qprod(acc, Qs, y, Dp, bi, bj, wave, lane, jac, zm(0, py));
+            qprod(acc, y, Qh, Dp, bi, bj, wave, lane, jac, zm(py, 0));
+            qprod(acc, v, x, Dp, bi, bj, wave, lane, jac, zm(pw ^ 1, px));
+            qcolumn(T1, y, Rh, nullptr, nullptr, Dp, bj, wave, lane, red, zt, py, 1);
+            qcolumn(T2, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red, zt, px, 1);
+            qprod_t(acc, R, T1, Dp, bi, wave, lane, jac, zm(1, py ^ 1));
+            qprod_t(acc, w, T2, Dp, bi, wave, lane, jac, zm(pw, px ^ 1));
         }
         if (k >= 1) {
             const cplx* yp = mat(1, k - 1);
             const cplx* xp = mat(0, k - 1);
-            qprod(acc, R, yp, Dp, bi, bj, wave, lane, k * jt);
-            qprod(acc, yp, Rh, Dp, bi, bj, wave, lane, k * jt);
-            qprod(acc, w, xp, Dp, bi, bj, wave, lane, k * jt);
+            qprod(acc, R, yp, Dp, bi, bj, wave, lane, k * jt, zm(1, py ^ 1));
+            qprod(acc, yp, Rh, Dp, bi, bj, wave, lane, k * jt, zm(py ^ 1, 1));
+            qprod(acc, w, xp, Dp, bi, bj, wave, lane, k * jt, zm(pw, px ^ 1));
         }
     } else if (blk == 2) {
         const cplx* a0 = mat(2, k);
         const cplx* a1 = mat(3, k);
         if (jac != 0.0) {
-            qprod(acc, Qhs, a0, Dp, bi, bj, wave, lane, jac);
-            qprod(acc, a0, Q, Dp, bi, bj, wave, lane, jac);
-            qprod(acc, a1, v, Dp, bi, bj, wave, lane, jac);
-            qcolumn(T1, a0, R, a1, w, Dp, bj, wave, lane, red);
-            qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac);
+            qprod(acc, Qhs, a0, Dp, bi, bj, wave, lane, jac, zm(0, py));
+            qprod(acc, a0, Q, Dp, bi, bj, wave, lane, jac, zm(py, 0));
+            qprod(acc, a1, v, Dp, bi, bj, wave, lane, jac, zm(px, pw ^ 1));
+            qcolumn(T1, a0, R, a1, w, Dp, bj, wave, lane, red, zt, py, 1, px, pw);
+            qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac, zm(1, py ^ 1));
         }
         if (k <= 3) {
             const double c = (k + 1) * jt;
             const cplx* a0n = mat(2, k + 1);
             const cplx* a1n = mat(3, k + 1);
-            qprod(acc, a0n, R, Dp, bi, bj, wave, lane, c);
-            qprod(acc, a1n, w, Dp, bi, bj, wave, lane, c);
-            qprod(acc, Rh, a0n, Dp, bi, bj, wave, lane, c);
+            qprod(acc, a0n, R, Dp, bi, bj, wave, lane, c, zm(py ^ 1, 1));
+            qprod(acc, a1n, w, Dp, bi, bj, wave, lane, c, zm(px ^ 1, pw));
+            qprod(acc, Rh, a0n, Dp, bi, bj, wave, lane, c, zm(1, py ^ 1));
         }
     } else {
         const cplx* a1 = mat(3, k);
         if (jac != 0.0) {
-            qprod(acc, Qh, a1, Dp, bi, bj, wave, lane, jac);
-            qprod(acc, a1, Q, Dp, bi, bj, wave, lane, jac);
-            qcolumn(T1, a1, R, nullptr, nullptr, Dp, bj, wave, lane, red);
-            qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac);
+            qprod(acc, Qh, a1, Dp, bi, bj, wave, lane, jac, zm(0, px));
+            qprod(acc, a1, Q, Dp, bi, bj, wave, lane, jac, zm(px, 0));
+            qcolumn(T1, a1, R, nullptr, nullptr, Dp, bj, wave, lane, red, zt, px, 1);
+            qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac, zm(1, px ^ 1));
         }
         if (k <= 3) {
             const double c = (k + 1) * jt;
             const cplx* a1n = mat(3, k + 1);
-            qprod(acc, a1n, R, Dp, bi, bj, wave, lane, c);
-            qprod(acc, Rh, a1n, Dp, bi, bj, wave, lane, c);
+            qprod(acc, a1n, R, Dp, bi, bj, wave, lane, c, zm(px ^ 1, 1));
+            qprod(acc, Rh, a1n, Dp, bi, bj, wave, lane, c, zm(1, px ^ 1));
         }
     }
     const cplx val = qreduce(acc, red, wave, lane);

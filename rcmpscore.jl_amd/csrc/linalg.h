@@ -30,6 +30,12 @@ __global__ void rc_axpby(int n, double ar, double ai, const double* X, double br
 __global__ void rc_herm(int Dp, int mode, double scale, const double* X, double* out);
 __global__ void rc_sylvester_diag(int Dp, int D, int adjoint, double shift, const double* lam, const double* Y,
                                   double* Z);
+__global__ void rc_colmajor_in(int D, int Dp, const double* src, double* dst);
+__global__ void rc_colmajor_out(int D, int Dp, const double* src, double* dst);
+__global__ void rc_sylvester_tri(int Dp, int D, int adjoint, double s_re, double s_im, const double* T,
+                                 const double* Y, double* Z);
+size_t rc_sylvester_tri_lds(int D);   // dynamic LDS bytes of rc_sylvester_tri (one workgroup of 256)
+#define RC_SYLVESTER_TRI_MAX_D 64
 __global__ void rc_dots(const double* V, long long stride, int n, const double* w, double* out);
 __global__ void rc_subtract_combo(const double* V, long long stride, int nk, const double* h, int n, double* w);
 __global__ void rc_add_combo(const double* V, long long stride, int nk, const double* y, int n, double* x);

@@ -372,6 +372,23 @@ __global__ void rc_herm(int Dp, int mode, double scale, const double* X, double*
     reinterpret_cast<cplx*>(out)[i] = v;
 }
 
+// Boundary copies of a column-major D x D matrix (element (i, j) at i + j D, Julia's layout):
+// into the padded row-major Dp x Dp working layout (zero padding), and back out.
+__global__ void rc_colmajor_in(int D, int Dp, const double* src, double* dst) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Dp * Dp) return;
+    const int i = e / Dp, j = e % Dp;
+    reinterpret_cast<cplx*>(dst)[e] =
+        (i < D && j < D) ? reinterpret_cast<const cplx*>(src)[(size_t)j * D + i] : cmake(0, 0);
+}
+
+__global__ void rc_colmajor_out(int D, int Dp, const double* src, double* dst) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= D * D) return;
+    const int i = e % D, j = e / D;   // dst element e = (i, j) column-major
+    reinterpret_cast<cplx*>(dst)[e] = reinterpret_cast<const cplx*>(src)[(size_t)i * Dp + j];
+}
+
 // Z_ij = -Y_ij / den_ij   (den = lam_i + conj(lam_j) - shift, or conj(lam_i) + lam_j - shift)
 __global__ void rc_sylvester_diag(int Dp, int D, int adjoint, double shift, const double* lam, const double* Y,
                                   double* Z) {
@@ -388,6 +405,87 @@ __global__ void rc_sylvester_diag(int Dp, int D, int adjoint, double shift, cons
     cplx q = cmake((y.re * den.re + y.im * den.im) / m, (y.im * den.re - y.re * den.im) / m);
     z[i] = cmake(-q.re, -q.im);
 }
+
+// The same Sylvester solve in the Schur basis of Q (Q = Z T Z^+, T upper triangular), for a Q whose
+// eigenvectors are ill-conditioned (defective or nearly so):
+//   forward  T X + X T^+ - s X = -Y,   adjoint  T^+ X + X T - s X = -Y,   s = s_re + i s_im,
+// by the Bartels-Stewart recurrence (src/transfer.jl:50-68 solves the same triangular system with
+// LAPACK's trsyl).  Forward, X_ij needs X_kj for k > i and X_ik for k > j; adjoint, k < i and k < j:
+// the elements of one anti-diagonal i + j = const are independent, so the sweep runs over the 2D - 1
+// anti-diagonals (from the bottom-right corner forward, from the top-left corner for the adjoint).
+// One workgroup of 256 threads, T and X in LDS (2 D^2 complex, D <= 64); four lanes per element
+// split both inner sums (k = part, part + 4, ...) and add them by shuffles.
+__global__ void __launch_bounds__(256) rc_sylvester_tri(int Dp, int D, int adjoint, double s_re, double s_im,
+                                                        const double* T, const double* Y, double* Z) {
+    extern __shared__ cplx syl_lds[];
+    cplx* sT = syl_lds;
+    cplx* sX = syl_lds + D * D;
+    const int tid = threadIdx.x;
+    const cplx* t = reinterpret_cast<const cplx*>(T);
+    const cplx* y = reinterpret_cast<const cplx*>(Y);
+    for (int e = tid; e < D * D; e += 256) {
+        sT[e] = t[(e / D) * Dp + e % D];
+        sX[e] = cmake(0, 0);
+    }
+    __syncthreads();
+    const int nd = 2 * D - 1, part = tid & 3;
+    for (int step = 0; step < nd; ++step) {
+        const int sd = adjoint ? step : nd - 1 - step;
+        const int i_lo = sd > D - 1 ? sd - (D - 1) : 0, i_hi = sd < D - 1 ? sd : D - 1;
+        for (int e0 = 0; e0 <= i_hi - i_lo; e0 += 64) {
+            const int e = e0 + (tid >> 2);
+            const bool live = e <= i_hi - i_lo;
+            const int i = i_lo + e, j = sd - i;
+            double re = 0.0, im = 0.0;
+            if (live) {
+                if (!adjoint) {
+                    for (int k = i + 1 + part; k < D; k += 4) {     // T_ik X_kj
+                        const cplx a = sT[i * D + k], b = sX[k * D + j];
+                        re += a.re * b.re - a.im * b.im;
+                        im += a.re * b.im + a.im * b.re;
+                    }
+                    for (int k = j + 1 + part; k < D; k += 4) {     // X_ik conj(T_jk)
+                        const cplx a = sX[i * D + k], b = sT[j * D + k];
+                        re += a.re * b.re + a.im * b.im;
+                        im += a.im * b.re - a.re * b.im;
+                    }
+                } else {
+                    for (int k = part; k < i; k += 4) {             // conj(T_ki) X_kj
+                        const cplx a = sT[k * D + i], b = sX[k * D + j];
+                        re += a.re * b.re + a.im * b.im;
+                        im += a.re * b.im - a.im * b.re;
+                    }
+                    for (int k = part; k < j; k += 4) {             // X_ik T_kj
+                        const cplx a = sX[i * D + k], b = sT[k * D + j];
+                        re += a.re * b.re - a.im * b.im;
+                        im += a.re * b.im + a.im * b.re;
+                    }
+                }
+            }
+            re += __shfl_xor(re, 1);
+            im += __shfl_xor(im, 1);
+            re += __shfl_xor(re, 2);
+            im += __shfl_xor(im, 2);
+            if (live && part == 0) {
+                const cplx ti = sT[i * D + i], tj = sT[j * D + j];
+                cplx den = adjoint ? cadd(cconj(ti), tj) : cadd(ti, cconj(tj));
+                den.re -= s_re;
+                den.im -= s_im;
+                const cplx yy = y[i * Dp + j];
+                const double nr = yy.re + re, ni = yy.im + im, m = cabs2(den);
+                sX[i * D + j] = cmake(-(nr * den.re + ni * den.im) / m, -(ni * den.re - nr * den.im) / m);
+            }
+        }
+        __syncthreads();
+    }
+    cplx* z = reinterpret_cast<cplx*>(Z);
+    for (int e = tid; e < Dp * Dp; e += 256) {
+        const int r = e / Dp, c = e % Dp;
+        z[e] = (r < D && c < D) ? sX[r * D + c] : cmake(0, 0);
+    }
+}
+
+size_t rc_sylvester_tri_lds(int D) { return (size_t)2 * D * D * sizeof(cplx); }
 
 // Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.  Thread t
 // adds elements t, t + 256, ... in increasing order, eight elements' loads issued together.

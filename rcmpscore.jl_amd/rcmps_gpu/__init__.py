@@ -2,7 +2,7 @@
 continuous MPS), its state preparation and optimizer, running on hand-written HIP kernels
 (librcmps_hip.so, C ABI in include/rcmps.h).  Importing this package does not touch the GPU.
 """
-from ._lib import DeviceControlFailure, SolverFailure, gemm_tuning, gemm_variant
+from ._lib import DeviceControlFailure, SolverFailure, gemm_tuning, gemm_variant, nojump_threshold
 from .autograd import Phi4Energy, lbfgs_rk, phi4_energy
 from .kinetic import KineticKrylovOptions, KineticOptions, kinetic1, kinetic2
 from .excitation import (DavidsonOptions, ExcitationContext, ExcitationModel, ExcitationOperator,
@@ -34,6 +34,6 @@ __all__ = [
     "action_dual_poisson", "davidson", "dispersion_relation", "excitation_eigsolve", "excitation_hamiltonian_action",
     "excitation_parity", "excitation_poisson", "field_seed", "iterative_dispersion", "kinetic_preconditioner",
     "relativistic_diagnostics", "excitation_element", "tangent_metric", "excitation_matrix_element",
-    "gemm_tuning", "gemm_variant", "controller", "OptimizationControls", "RobustnessControls", "gpu_objective",
+    "gemm_tuning", "gemm_variant", "nojump_threshold", "controller", "OptimizationControls", "RobustnessControls", "gpu_objective",
     "gpu_stationary", "load_resume_state", "save_resume_state",
 ]

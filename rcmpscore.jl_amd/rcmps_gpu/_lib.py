@@ -62,7 +62,8 @@ class Stats(ctypes.Structure):
                 ("krylov_matvecs", ctypes.c_int), ("krylov_residual", ctypes.c_double),
                 ("phase_us", (ctypes.c_double * 12) * 6),
                 ("kinetic_nodes", ctypes.c_int), ("kinetic_basis", ctypes.c_int * 4),
-                ("kinetic_actions", ctypes.c_int), ("kinetic_residual", ctypes.c_double)]
+                ("kinetic_actions", ctypes.c_int), ("kinetic_residual", ctypes.c_double),
+                ("kinetic_ms", ctypes.c_double * 4)]
 
     def as_dict(self):
         out = {}
@@ -103,6 +104,9 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_P), ctypes.POINTER(Stats)]),
     "rcmps_set_kinetic_options": (ctypes.c_int, [_P, ctypes.POINTER(KineticOptions)]),
     "rcmps_gemm_variant": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rcmps_set_nojump_threshold": (ctypes.c_int, [_P, ctypes.c_double]),
+    "rcmps_set_layout": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rcmps_state_nojump_form": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
     "rcmps_gemm_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_int)]),
@@ -131,6 +135,7 @@ SIGNATURES = {
 }
 EXC_KINETIC = 1
 EXC_POTENTIAL = 2
+EXC_DENSE = 4
 
 _lock = threading.Lock()
 _lib = None
@@ -221,6 +226,14 @@ def gemm_variant(variant: int | None = None, device: int = 0):
     to the autotuner (None).  Every variant produces the same bits."""
     v = -1 if variant is None else int(variant)
     check(load().rcmps_gemm_variant(context(device), v), "gemm variant")
+
+
+def nojump_threshold(kappa: float = 1e6, device: int = 0):
+    """States created afterwards on the device apply the no-jump inverse in the Schur basis of Q when
+    the condition estimate of its eigenvectors exceeds kappa (0: always; inf: never), else in the
+    eigenbasis.  The default 1e6 leaves the eigenbasis to every diagonalisable Q met in practice and
+    the Schur form to defective ones."""
+    check(load().rcmps_set_nojump_threshold(context(device), float(kappa)), "nojump threshold")
 
 
 def gemm_tuning(device: int = 0):

@@ -110,6 +110,15 @@ class State:
     def handle(self):
         return self._handle
 
+    @property
+    def nojump_form(self):
+        """("schur" | "eigen", kappa): the basis the no-jump preconditioner of this state applies in,
+        and the condition estimate of Q's eigenvectors that chose it (rcmps_gpu.nojump_threshold)."""
+        schur, kappa = ctypes.c_int(), ctypes.c_double()
+        _lib.check(_lib.load().rcmps_state_nojump_form(self._handle, ctypes.byref(schur), ctypes.byref(kappa)),
+                   "nojump form")
+        return ("schur" if schur.value else "eigen"), kappa.value
+
     def close(self):
         if self._handle is not None and not _lib.closed():
             try:
