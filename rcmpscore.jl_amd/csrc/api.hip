@@ -1580,34 +1580,48 @@ void basis_start(rcmps_context* ctx, Dev& dv, KrylovBasis& b, const double* rhs,
 
 // Arnoldi steps up to `target` vectors, classical Gram-Schmidt twice, H from both passes
 // (kinetic_krylov.jl:68-92); exhaustion at a vanishing new direction or the full dimension D^2.
+// No step waits for the host: each writes its Hessenberg column (|w|^2 before, the two passes'
+// projections, |w|^2 after) to its own slot and normalises on the device, and the host reads all
+// columns once.  A step past an exhaustion divides by a vanishing norm; its vector and column are
+// discarded (the basis keeps the vectors up to the exhaustion), so the kept bits are those of a
+// step-by-step loop.
 void basis_extend(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasis& b, bool adj, int target) {
     KineticWork& kw = ctx->kin;
     const int n = dv.n();
+    const int D2 = st->D * st->D;
+    target = std::min(target, D2);
     basis_grow(b, n, target, ctx->stream);
-    ck(kw.hd.ensure((size_t)(4 * (b.cap + 1) + 8) * 8));
+    if (target <= b.k) return;
+    const int nsteps = target - b.k;
+    const size_t stride = 2 * (size_t)(b.cap + 1) + 2;   // complex entries per step's column slot
+    ck(kw.hd.ensure((size_t)nsteps * stride * 16));
     double* V = b.V.d();
     double* w = ctx->kry_w.d();
-    double* hd0 = kw.hd.d();
-    double* hd1 = hd0 + 2;
-    double* hd2 = hd1 + 2 * (b.cap + 1);
-    double* hnn = hd2 + 2 * (b.cap + 1);
-    std::vector<cd> hv(2 * (b.cap + 1) + 2);
-    const int D2 = st->D * st->D;
     for (int j = b.k; j < target; ++j) {
+        double* hd0 = kw.hd.d() + 2 * (size_t)(j - b.k) * stride;
+        double* hd1 = hd0 + 2;
+        double* hd2 = hd1 + 2 * (b.cap + 1);
+        double* hnn = hd2 + 2 * (b.cap + 1);
         lindblad(dv, st, V + (size_t)j * n * 2, w, adj, ctx->kry_t1.d());
-        ++kw.actions;
         hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hd0);
         hipLaunchKernelGGL(rc_dots, dim3(j + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd1);
         hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, j + 1, hd1, n, w);
         hipLaunchKernelGGL(rc_dots, dim3(j + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd2);
         hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, j + 1, hd2, n, w);
         hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hnn);
+        hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(j + 1) * n * 2);
         ck(hipGetLastError());
-        ck(hipMemcpyAsync(hv.data(), hd0, (size_t)(4 * (b.cap + 1) + 4) * 8, hipMemcpyDeviceToHost, ctx->stream));
-        ck(hipStreamSynchronize(ctx->stream));
-        const double oldnorm = std::sqrt(std::max(0.0, hv[0].real()));
-        for (int i = 0; i <= j; ++i) b.H[(size_t)i * b.cap + j] += hv[1 + i] + hv[1 + (b.cap + 1) + i];
-        const double beta = std::sqrt(std::max(0.0, hv[1 + 2 * (b.cap + 1)].real()));
+    }
+    std::vector<cd> hv((size_t)nsteps * stride);
+    ck(hipMemcpyAsync(hv.data(), kw.hd.d(), hv.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+    ck(hipStreamSynchronize(ctx->stream));
+    const int k0 = b.k;
+    for (int j = k0; j < target; ++j) {
+        const cd* col = hv.data() + (size_t)(j - k0) * stride;
+        ++kw.actions;
+        const double oldnorm = std::sqrt(std::max(0.0, col[0].real()));
+        for (int i = 0; i <= j; ++i) b.H[(size_t)i * b.cap + j] += col[1 + i] + col[1 + (b.cap + 1) + i];
+        const double beta = std::sqrt(std::max(0.0, col[1 + 2 * (b.cap + 1)].real()));
         if (!std::isfinite(oldnorm) || !std::isfinite(beta)) throw Fail{RCMPS_ERR_SOLVER};
         b.H[(size_t)(j + 1) * b.cap + j] = beta;
         b.k = j + 1;
@@ -1615,8 +1629,6 @@ void basis_extend(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasi
             b.exhausted = true;
             break;
         }
-        hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(j + 1) * n * 2);
-        ck(hipGetLastError());
     }
 }
 
