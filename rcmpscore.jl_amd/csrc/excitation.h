@@ -35,6 +35,7 @@ struct RcQuarticArgs {
     // every tangent of the launch of parity wpar (0 even, 1 odd), each channel has a definite parity
     // and the products run only the k-tiles of the one parity block that can be nonzero
     int zt, wpar;
+    const int* active;    // device flag: the launch does nothing when it reads 0 (null: always runs)
 };
 
 __global__ void rc_quartic_rhs(RcQuarticArgs g);
@@ -76,7 +77,26 @@ __global__ void rc_msubtract(const double* V, long long sb, long long sv, int nk
 __global__ void rc_mnormalize(int n, const double* W, long long sw, const double* nrm2, int ldn, double* out,
                               long long so);
 __global__ void rc_stage(long long N, const double* y, const double* K, long long sk, RcStageCoef c, double h,
-                         const double* hp, double* out);
+                         const double* hp, double* out, const int* active);
+
+// Device-resident DOP853 step controller of the action ODE.  One attempt is rc_ode_stage (which
+// first commits the previous attempt when it was accepted: y <- y_new, K_0 <- K_12, the sample of a
+// reached quadrature node; then forms the first stage input), the right-hand sides, and rc_ode_step
+// (error norm, accept or reject, the next step size, then the next attempt's step clipped at the next
+// stop and its stage coefficients).  The host replays attempts without reading anything back until
+// `done`; attempts replayed after the end do nothing but the (idempotent) last commit.
+struct RcOdeCtl {
+    double t, h, he, tn;
+    long long attempts, accepted, rejected, maxiters;
+    int si, node, nstops, npts;
+    int done, active, commit, sample, last_rejected, clipped, status, pad;
+};
+__global__ void rc_ode_step(RcOdeCtl* c, int finish, const double* partial, const int* ranges, int nranges,
+                            int ntrack, double cnt, const double* stops, const int* stop_node, const double* cheb,
+                            double* cf);
+__global__ void rc_ode_stage(const RcOdeCtl* c, long long N, long long half, double a10, const double* hp,
+                             const double* ynew, double* y, const double* K12, double* K0, double* Sf, double* Sa,
+                             double* U1);
 __global__ void rc_trace_prod_batched(int Dp, const double* A, long long sa, const double* B, long long sb,
                                       double* out);
 __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, double* X);

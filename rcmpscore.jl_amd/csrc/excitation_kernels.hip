@@ -3,6 +3,7 @@
 // action ODEs, DOP853 stage combinations and error norms, and batched trace products.
 #include "device_common.h"
 #include "excitation.h"
+#include "trajectory.h"
 
 // Z_ij = -Y_ij / den_ij with den = lam_i + conj(lam_j) - (s_re + i s_im) (forward) or
 // conj(lam_i) + lam_j - (s_re + i s_im) (adjoint)
@@ -88,9 +89,9 @@ __global__ void rc_mnormalize(int n, const double* W, long long sw, const double
 
 // out = y + h sum_{j < c.nk} c.a[j] K_j  (N complex, K_j at K + j sk); h from hp when non-null
 __global__ void rc_stage(long long N, const double* y, const double* K, long long sk, RcStageCoef c, double h,
-                         const double* hp, double* out) {
+                         const double* hp, double* out, const int* active) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i >= N || (active && !*active)) return;
     if (hp) h = hp[0];
     const cplx* kk = reinterpret_cast<const cplx*>(K);
     cplx s = cmake(0, 0);
@@ -98,6 +99,155 @@ __global__ void rc_stage(long long N, const double* y, const double* K, long lon
         if (c.a[j] != 0.0) s = cadd(s, cscale(c.a[j], kk[(long long)j * sk + i]));
     const cplx yy = reinterpret_cast<const cplx*>(y)[i];
     reinterpret_cast<cplx*>(out)[i] = cadd(yy, cscale(h, s));
+}
+
+// ---------------------------------------------------------------------------------------------
+// The action ODE's step controller on the device (the host loop it replaces: SciPy's DOP853
+// controller as the reference's action integration uses it, src/excitation_action.jl:56-123).
+// ---------------------------------------------------------------------------------------------
+// rc_ode_step, one workgroup of 256.  finish = 1: judge the attempt that just ran (when active): per
+// tracked trajectory its tiles' E5 and E3 sums (slot ranges ranges[tr][r] = (first, count), one wave
+// per trajectory, lane-strided with a fixed shuffle tree), |h| e5 / sqrt((e5 + e3 / 100) cnt), the
+// worst over trajectories; accept or reject and the next step size.  Then, unless done, set up the
+// next attempt: the step clipped at the next stop (a quadrature node, 0 or the window's end) and the
+// source coefficients [2|t_s|, s(t_s)] of stages 1..12 (one thread each) with the step at cf[26].
+__global__ void __launch_bounds__(256) rc_ode_step(RcOdeCtl* c, int finish, const double* partial,
+                                                   const int* ranges, int nranges, int ntrack, double cnt,
+                                                   const double* stops, const int* stop_node, const double* cheb,
+                                                   double* cf) {
+    __shared__ double smax[4];
+    __shared__ int sbad[4];
+    __shared__ double s_t, s_he, s_tn;
+    __shared__ int s_go;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool judge = finish && c->active;   // uniform
+    if (judge) {
+        const double he = c->he;
+        double worst = 0.0;
+        int bad = 0;
+        for (int tr = wave; tr < ntrack; tr += 4) {
+            double e5 = 0.0, e3 = 0.0;
+            for (int r = 0; r < nranges; ++r) {
+                const int first = ranges[2 * (tr * nranges + r)], count = ranges[2 * (tr * nranges + r) + 1];
+                for (int q = lane; q < count; q += 64) {
+                    e5 += partial[2 * (first + q)];
+                    e3 += partial[2 * (first + q) + 1];
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                e5 += __shfl_xor(e5, off);
+                e3 += __shfl_xor(e3, off);
+            }
+            double et = 0.0;
+            if (e5 > 0 || e3 > 0) et = fabs(he) * e5 / sqrt((e5 + 0.01 * e3) * cnt);
+            if (!isfinite(et)) bad = 1;
+            else worst = fmax(worst, et);
+        }
+        if (lane == 0) {
+            smax[wave] = worst;
+            sbad[wave] = bad;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        s_go = 0;
+        if (judge) {
+            const double err = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+            const int bad = sbad[0] | sbad[1] | sbad[2] | sbad[3];
+            const double he = c->he;
+            c->commit = 0;
+            c->sample = -1;
+            if (bad) {
+                c->status = RC_ST_NONFINITE;
+                c->done = 1;
+            } else if (err <= 1.0) {
+                c->accepted += 1;
+                c->commit = 1;
+                c->t = c->tn;
+                if (c->t == stops[c->si]) {
+                    const int nd = stop_node[c->si];
+                    if (nd >= 0) {
+                        c->sample = nd;
+                        c->node = nd + 1;
+                    }
+                    c->si += 1;
+                }
+                double factor = err == 0 ? 10.0 : fmin(10.0, 0.9 * pow(err, -1.0 / 8));
+                if (c->last_rejected) factor = fmin(1.0, factor);
+                const double hn = he * factor;
+                c->h = c->clipped ? fmax(hn, c->h) : hn;
+                c->last_rejected = 0;
+                if (c->si >= c->nstops) {
+                    c->done = 1;
+                    if (c->node != c->npts) c->status = RC_ST_MISSING_SAMPLES;
+                }
+            } else {
+                c->rejected += 1;
+                c->h = he * fmax(0.2, 0.9 * pow(err, -1.0 / 8));
+                c->last_rejected = 1;
+            }
+        }
+        c->active = 0;
+        if (!c->done) {
+            const double t = c->t, h = c->h, stop = stops[c->si];
+            double he = fmin(h, stop - t);
+            const int clipped = he < h;
+            if (stop - t - he <= 1e-14 * fmax(1.0, fabs(stop))) he = stop - t;
+            if (!(he > 10 * 2.220446049250313e-16 * fmax(1.0, fabs(t)))) {
+                c->status = RC_ST_STEP_UNDERFLOW;
+                c->done = 1;
+            } else if (++c->attempts > c->maxiters) {
+                c->status = RC_ST_MAX_STEPS;
+                c->done = 1;
+            } else {
+                const double tn = (he == stop - t) ? stop : t + he;
+                c->he = he;
+                c->tn = tn;
+                c->clipped = clipped;
+                c->active = 1;
+                s_t = t;
+                s_he = he;
+                s_tn = tn;
+                s_go = 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (s_go && tid >= 1 && tid < 13) {
+        const double ts = tid < 12 ? s_t + RC_C[tid] * s_he : s_tn;
+        cf[2 * tid] = 2 * fabs(ts);
+        cf[2 * tid + 1] = rc_source(ts, cheb);
+    }
+    if (s_go && tid == 0) cf[26] = s_he;
+}
+
+// First stage input of the attempt, U1 = y + h a10 K_0, preceded by the commit of the previous
+// attempt when it was accepted (y <- y_new, K_0 <- K_12, and at a quadrature node the forward half
+// of y_new as sample `sample`, the adjoint half as sample npts - 1 - sample).
+__global__ void rc_ode_stage(const RcOdeCtl* c, long long N, long long half, double a10, const double* hp,
+                             const double* ynew, double* y, const double* K12, double* K0, double* Sf, double* Sa,
+                             double* U1) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int commit = c->commit, active = c->active;
+    cplx yi, ki;
+    if (commit) {
+        yi = reinterpret_cast<const cplx*>(ynew)[i];
+        ki = reinterpret_cast<const cplx*>(K12)[i];
+        reinterpret_cast<cplx*>(y)[i] = yi;
+        reinterpret_cast<cplx*>(K0)[i] = ki;
+        const int smp = c->sample;
+        if (smp >= 0) {
+            if (i < half) reinterpret_cast<cplx*>(Sf)[(long long)smp * half + i] = yi;
+            else reinterpret_cast<cplx*>(Sa)[(long long)(c->npts - 1 - smp) * half + (i - half)] = yi;
+        }
+    } else {
+        if (!active) return;
+        yi = reinterpret_cast<const cplx*>(y)[i];
+        ki = reinterpret_cast<const cplx*>(K0)[i];
+    }
+    if (active) reinterpret_cast<cplx*>(U1)[i] = cadd(yi, cscale(hp[0], cscale(a10, ki)));
 }
 
 // out[b] = tr(A_b B_b) for Dp x Dp matrices; grid (nb)
@@ -225,6 +375,7 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
     __shared__ cplx red[1024];
     __shared__ cplx T1[64 * 16];
     __shared__ cplx T2[64 * 16];
+    if (g.active && !*g.active) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int Dp = g.Dp, nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
     const int nb = g.na * 5, mi = blockIdx.y, blk = mi / nb, r = mi % nb, a = r / 5, k = r % 5;
