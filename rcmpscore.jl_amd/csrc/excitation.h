@@ -65,6 +65,7 @@ struct RcPairArgs {
     const double* hp;
     double h, atol, rtol;
     double* partial;
+    const int* active;    // as RcQuarticArgs::active
 };
 
 __global__ void rc_pair_rhs(RcPairArgs g);

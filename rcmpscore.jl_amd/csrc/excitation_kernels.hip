@@ -353,18 +353,48 @@ __device__ __forceinline__ cplx qreduce(const cacc& acc, cplx* red, int wave, in
     return s;
 }
 
-// T = X1 B1[:, tile] (+ X2 B2[:, tile]) as a Dp x 16 block in LDS; parities p1a, p1b (p2a, p2b) of the
-// operands for the Z2 masks
-__device__ __forceinline__ void qcolumn(cplx* T, const cplx* X1, const cplx* B1, const cplx* X2, const cplx* B2,
-                                        int Dp, int bj, int wave, int lane, cplx* red, int zt = 0, int p1a = 0,
-                                        int p1b = 0, int p2a = 0, int p2b = 0) {
-    for (int rt = 0; rt < Dp / 16; ++rt) {
+// One wave's whole product A(bi, :) B(:, bj) over every k-step (no cross-wave reduction), one k-tile of
+// fragments loaded ahead of its four MFMAs; kmask selects the k-tiles (bit t: k-steps 4t .. 4t + 3).
+__device__ __forceinline__ void qprod_rows(cacc& acc, const cplx* A, const cplx* B, int Dp, int bi, int bj, int lane,
+                                           int kmask) {
+    const int nk = Dp / 16;
+    for (int t = 0; t < nk; ++t) {
+        if (!((kmask >> t) & 1)) continue;
+        cplx a[4], b[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int kk = 16 * t + 4 * s + (lane >> 4);
+            a[s] = A[(16 * bi + (lane & 15)) * Dp + kk];
+            b[s] = B[kk * Dp + 16 * bj + (lane & 15)];
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) cmfma(acc, a[s], b[s]);
+    }
+}
+
+// one term X B[:, bj] of an LDS column block, with the operands' Z2 parities (X null: no term)
+struct QTerm {
+    const cplx* X;
+    const cplx* B;
+    int pa, pb;
+};
+
+// Column blocks (Dp x 16, in LDS) T0 = a0 + a1 and, when T1 is given, T1 = b0 + b1.  The row tiles of
+// both blocks are spread over the waves, each wave running every k-step of its row tiles and writing
+// them straight to LDS; one barrier at the end.  zt: Z2 parity blocks (0: every k-tile).
+__device__ __forceinline__ void qcolumns(cplx* T0, QTerm a0, QTerm a1, cplx* T1, QTerm b0, QTerm b1, int Dp, int bj,
+                                         int wave, int lane, int zt) {
+    const int nbt = Dp / 16, ntask = (T1 ? 2 : 1) * nbt;
+    for (int i = wave; i < ntask; i += 4) {
+        const int col = i / nbt, rt = i % nbt;
+        const QTerm p = col ? b0 : a0, q = col ? b1 : a1;
+        cplx* T = col ? T1 : T0;
         cacc acc;
         cacc_zero(acc);
-        qprod(acc, X1, B1, Dp, rt, bj, wave, lane, 1.0, zmask(zt, Dp / 16, rt, bj, p1a, p1b));
-        if (X2) qprod(acc, X2, B2, Dp, rt, bj, wave, lane, 1.0, zmask(zt, Dp / 16, rt, bj, p2a, p2b));
-        const cplx v = qreduce(acc, red, wave, lane);
-        T[(16 * rt + (threadIdx.x >> 4)) * 16 + (threadIdx.x & 15)] = v;
+        qprod_rows(acc, p.X, p.B, Dp, rt, bj, lane, zmask(zt, nbt, rt, bj, p.pa, p.pb));
+        if (q.X) qprod_rows(acc, q.X, q.B, Dp, rt, bj, lane, zmask(zt, nbt, rt, bj, q.pa, q.pb));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) T[(16 * rt + 4 * e + (lane >> 4)) * 16 + (lane & 15)] = cacc_get(acc, e);
     }
     __syncthreads();
 }
@@ -405,7 +435,7 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
         if (jac != 0.0) {
             qprod(acc, Q, x, Dp, bi, bj, wave, lane, jac, zm(0, px));
             qprod(acc, x, Qh, Dp, bi, bj, wave, lane, jac, zm(px, 0));
-            qcolumn(T1, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red, zt, px, 1);
+            qcolumns(T1, QTerm{x, Rh, px, 1}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac, zm(1, px ^ 1));
         }
         if (k >= 1) {
@@ -420,8 +450,7 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
             qprod(acc, Qs, y, Dp, bi, bj, wave, lane, jac, zm(0, py));
             qprod(acc, y, Qh, Dp, bi, bj, wave, lane, jac, zm(py, 0));
             qprod(acc, v, x, Dp, bi, bj, wave, lane, jac, zm(pw ^ 1, px));
-            qcolumn(T1, y, Rh, nullptr, nullptr, Dp, bj, wave, lane, red, zt, py, 1);
-            qcolumn(T2, x, Rh, nullptr, nullptr, Dp, bj, wave, lane, red, zt, px, 1);
+            qcolumns(T1, QTerm{y, Rh, py, 1}, QTerm{}, T2, QTerm{x, Rh, px, 1}, QTerm{}, Dp, bj, wave, lane, zt);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac, zm(1, py ^ 1));
             qprod_t(acc, w, T2, Dp, bi, wave, lane, jac, zm(pw, px ^ 1));
         }
@@ -439,7 +468,7 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
             qprod(acc, Qhs, a0, Dp, bi, bj, wave, lane, jac, zm(0, py));
             qprod(acc, a0, Q, Dp, bi, bj, wave, lane, jac, zm(py, 0));
             qprod(acc, a1, v, Dp, bi, bj, wave, lane, jac, zm(px, pw ^ 1));
-            qcolumn(T1, a0, R, a1, w, Dp, bj, wave, lane, red, zt, py, 1, px, pw);
+            qcolumns(T1, QTerm{a0, R, py, 1}, QTerm{a1, w, px, pw}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt);
             qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac, zm(1, py ^ 1));
         }
         if (k <= 3) {
@@ -455,7 +484,7 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
         if (jac != 0.0) {
             qprod(acc, Qh, a1, Dp, bi, bj, wave, lane, jac, zm(0, px));
             qprod(acc, a1, Q, Dp, bi, bj, wave, lane, jac, zm(px, 0));
-            qcolumn(T1, a1, R, nullptr, nullptr, Dp, bj, wave, lane, red, zt, px, 1);
+            qcolumns(T1, QTerm{a1, R, px, 1}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt);
             qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac, zm(1, px ^ 1));
         }
         if (k <= 3) {
@@ -517,6 +546,7 @@ __global__ void __launch_bounds__(256) rc_pair_rhs(RcPairArgs g) {
     __shared__ cplx red[1024];
     __shared__ cplx T1[64 * 16];
     __shared__ cplx T2[64 * 16];
+    if (g.active && !*g.active) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int Dp = g.Dp, nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
     const int per = 4 * g.nch, mi = blockIdx.y, p = mi / per, r = mi % per, c = r / 4, k = r % 4;
@@ -544,23 +574,22 @@ __global__ void __launch_bounds__(256) rc_pair_rhs(RcPairArgs g) {
         qprod(acc, k == 1 ? Qs : k == 2 ? Qb : Q, xk, Dp, bi, bj, wave, lane, jac);
         qprod(acc, xk, Qh, Dp, bi, bj, wave, lane, jac);
         if (k == 0) {
-            qcolumn(T1, xk, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qcolumns(T1, QTerm{xk, Rh, 0, 0}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, 0);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
         } else if (k == 1) {
             qprod(acc, v, x(c, 0), Dp, bi, bj, wave, lane, jac);
-            qcolumn(T1, xk, Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
-            qcolumn(T2, x(c, 0), Rh, nullptr, nullptr, Dp, bj, wave, lane, red);
+            qcolumns(T1, QTerm{xk, Rh, 0, 0}, QTerm{}, T2, QTerm{x(c, 0), Rh, 0, 0}, QTerm{}, Dp, bj, wave, lane, 0);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
             qprod_t(acc, w, T2, Dp, bi, wave, lane, jac);
         } else if (k == 2) {
             qprod(acc, x(c, 0), vbh, Dp, bi, bj, wave, lane, jac);
-            qcolumn(T1, xk, Rh, x(c, 0), uh, Dp, bj, wave, lane, red);
+            qcolumns(T1, QTerm{xk, Rh, 0, 0}, QTerm{x(c, 0), uh, 0, 0}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, 0);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
         } else {
             qprod(acc, v, x(c, 2), Dp, bi, bj, wave, lane, jac);
             qprod(acc, x(c, 1), vbh, Dp, bi, bj, wave, lane, jac);
-            qcolumn(T1, xk, Rh, x(c, 1), uh, Dp, bj, wave, lane, red);
-            qcolumn(T2, x(c, 2), Rh, x(c, 0), uh, Dp, bj, wave, lane, red);
+            qcolumns(T1, QTerm{xk, Rh, 0, 0}, QTerm{x(c, 1), uh, 0, 0}, T2, QTerm{x(c, 2), Rh, 0, 0},
+                     QTerm{x(c, 0), uh, 0, 0}, Dp, bj, wave, lane, 0);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac);
             qprod_t(acc, w, T2, Dp, bi, wave, lane, jac);
         }
