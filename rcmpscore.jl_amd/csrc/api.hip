@@ -223,6 +223,7 @@ struct rcmps_state {
     // the parity blocks are whole 16 x 16 tiles, z2_tiles = D / 32 (tiles per parity block), else 0.
     int z2_tiles = 0;
     DevBuf R, K, Q, Qh, Rh, C, Ch, rho, V, Vinv, lam, ident;
+    DevBuf VH, VinvH;   // V^+ and V^-+ for the fused two-sided products of the eigenbasis form
     // no-jump inverse: eigenbasis (V = eigenvectors of Q, Vinv their inverse, lam the eigenvalues) or,
     // when nojump_schur, the Schur basis (V = Z, Vinv = Z^+, Tq = T upper triangular, Q = Z T Z^+)
     int nojump_schur = 0;
@@ -479,6 +480,18 @@ void nojump_middle(Dev& dv, const rcmps_state* st, bool adjoint, double s_re, do
 
 void nojump_apply_c(Dev& dv, const rcmps_state* st, const double* Y, double* out, bool adjoint, double sigma,
                     double* t1, double* t2) {
+    if (!st->nojump_schur && dv.Dp <= 64) {
+        // two fused launches: (V^-1 Y V^-+ / den) then V (.) V^+ (adjoint: (V^+ Y V / den), V^-+ (.) V^-1)
+        const int nbt = dv.Dp / 16;
+        const double *a1 = adjoint ? st->VH.d() : st->Vinv.d(), *b1 = adjoint ? st->V.d() : st->VinvH.d();
+        const double *a2 = adjoint ? st->VinvH.d() : st->V.d(), *b2 = adjoint ? st->Vinv.d() : st->VH.d();
+        hipLaunchKernelGGL(rc_two_sided, dim3(nbt * nbt), dim3(256), 0, dv.ctx->stream, dv.Dp, dv.D, a1, Y, b1, t1, 1,
+                           adjoint ? 1 : 0, (const double*)st->lam.d(), st->shift, adjoint ? -sigma : sigma);
+        hipLaunchKernelGGL(rc_two_sided, dim3(nbt * nbt), dim3(256), 0, dv.ctx->stream, dv.Dp, dv.D, a2,
+                           (const double*)t1, b2, out, 0, 0, (const double*)nullptr, 0.0, 0.0);
+        ck(hipGetLastError());
+        return;
+    }
     if (!adjoint) {
         dv.gemm(0, st->Vinv.d(), 0, Y, t1);
         dv.gemm(0, t1, 1, st->Vinv.d(), t2);
@@ -501,6 +514,13 @@ void nojump_apply(Dev& dv, const rcmps_state* st, const double* Y, double* out, 
 
 // L(X) = Q X + X Q^+ + R X R^+ ; L*(X) = Q^+ X + X Q + R^+ X R
 void lindblad(Dev& dv, const rcmps_state* st, const double* X, double* out, bool adjoint, double* t1) {
+    if (dv.Dp <= 64) {   // one fused launch (rc_lindblad); four products otherwise
+        const int nbt = dv.Dp / 16;
+        hipLaunchKernelGGL(rc_lindblad, dim3(nbt * nbt), dim3(256), 0, dv.ctx->stream, dv.Dp, adjoint ? 1 : 0,
+                           st->Q.d(), st->Qh.d(), st->R.d(), st->Rh.d(), X, out);
+        ck(hipGetLastError());
+        return;
+    }
     if (!adjoint) {
         dv.gemm(0, st->Q.d(), 0, X, out);
         dv.gemm(0, X, 0, st->Qh.d(), out, 1.0, 1.0);
@@ -684,6 +704,10 @@ void state_spectral(rcmps_context* ctx, rcmps_state* st) {
     ck(st->lam.ensure((size_t)Dp * 16));
     upload(ctx, D, Dp, V, st->V.d());
     upload(ctx, D, Dp, Vinv, st->Vinv.d());
+    ck(st->VH.ensure((size_t)Dp * Dp * 16));
+    ck(st->VinvH.ensure((size_t)Dp * Dp * 16));
+    upload(ctx, D, Dp, rc::adjoint(D, V), st->VH.d());
+    upload(ctx, D, Dp, rc::adjoint(D, Vinv), st->VinvH.d());
     ck(hipMemcpyAsync(st->lam.d(), lam.data(), lam.size() * 8, hipMemcpyHostToDevice, ctx->stream));
     ck(hipStreamSynchronize(ctx->stream));
 }
@@ -837,7 +861,7 @@ void rcmps_context_destroy(rcmps_context* ctx) {
 void rcmps_state_destroy(rcmps_state* st) {
     if (!st) return;
     DevBuf* all[] = {&st->R, &st->K, &st->Q, &st->Qh, &st->Rh, &st->C, &st->Ch, &st->rho,
-                     &st->V, &st->Vinv, &st->lam, &st->ident, &st->Minv, &st->Tq};
+                     &st->V, &st->Vinv, &st->lam, &st->ident, &st->Minv, &st->Tq, &st->VH, &st->VinvH};
     for (auto* b : all) b->release();
     delete st;
 }

@@ -69,6 +69,10 @@ struct RcPairArgs {
 };
 
 __global__ void rc_pair_rhs(RcPairArgs g);
+__global__ void rc_lindblad(int Dp, int adjoint, const double* Q, const double* Qh, const double* R, const double* Rh,
+                            const double* X, double* out);
+__global__ void rc_two_sided(int Dp, int D, const double* A, const double* X, const double* B, double* out, int sylvester,
+                             int adjoint, const double* lam, double s_re, double s_im);
 __global__ void rc_sylvester_diag_c(int Dp, int D, int adjoint, double s_re, double s_im, const double* lam,
                                     const double* Y, double* Z);
 __global__ void rc_mdots(const double* V, long long sb, long long sv, int nk, const double* W, long long sw, int n,

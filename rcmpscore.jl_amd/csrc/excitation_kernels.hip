@@ -532,6 +532,69 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// L(X) = Q X + X Q^+ + R X R^+ (adjoint: L*(X) = Q^+ X + X Q + R^+ X R, src/transfer.jl:2-19) of
+// one Dp x Dp matrix in one launch: one workgroup per 16 x 16 output tile, the one-sided products
+// k-split over the waves into one accumulator, the two-sided term through the LDS column block
+// T = X R^+[:, bj] (adjoint X R[:, bj]).  Dp <= 64; out must not alias X.  grid (Dp/16)^2.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rc_lindblad(int Dp, int adjoint, const double* Q, const double* Qh,
+                                                   const double* R, const double* Rh, const double* X,
+                                                   double* out) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T[64 * 16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
+    const cplx* x = reinterpret_cast<const cplx*>(X);
+    const cplx* a1 = reinterpret_cast<const cplx*>(adjoint ? Qh : Q);    // a1 X
+    const cplx* b2 = reinterpret_cast<const cplx*>(adjoint ? Q : Qh);    // X b2
+    const cplx* c = reinterpret_cast<const cplx*>(adjoint ? R : Rh);     // T = X c[:, bj]
+    const cplx* l = reinterpret_cast<const cplx*>(adjoint ? Rh : R);     // l T
+    qcolumns(T, QTerm{x, c, 0, 0}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, 0);
+    cacc acc;
+    cacc_zero(acc);
+    qprod(acc, a1, x, Dp, bi, bj, wave, lane, 1.0);
+    qprod(acc, x, b2, Dp, bi, bj, wave, lane, 1.0);
+    qprod_t(acc, l, T, Dp, bi, wave, lane, 1.0);
+    const cplx v = qreduce(acc, red, wave, lane);
+    reinterpret_cast<cplx*>(out)[(long long)(16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15)] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// out = A X B in one launch (one workgroup per 16 x 16 output tile, X B[:, bj] as an LDS column
+// block), optionally followed by the no-jump inverse's elementwise step in the eigenbasis of Q:
+// out_ij <- -out_ij / den_ij, den = lam_i + conj(lam_j) - s (adjoint: conj(lam_i) + lam_j - s),
+// s = s_re + i s_im, zero outside the D x D block.  Dp <= 64; out must not alias X.  grid (Dp/16)^2.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rc_two_sided(int Dp, int D, const double* A, const double* X, const double* B,
+                                                    double* out, int sylvester, int adjoint, const double* lam,
+                                                    double s_re, double s_im) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T[64 * 16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
+    qcolumns(T, QTerm{reinterpret_cast<const cplx*>(X), reinterpret_cast<const cplx*>(B), 0, 0}, QTerm{}, nullptr,
+             QTerm{}, QTerm{}, Dp, bj, wave, lane, 0);
+    cacc acc;
+    cacc_zero(acc);
+    qprod_t(acc, reinterpret_cast<const cplx*>(A), T, Dp, bi, wave, lane, 1.0);
+    cplx v = qreduce(acc, red, wave, lane);
+    const int r = 16 * bi + (tid >> 4), c = 16 * bj + (tid & 15);
+    if (sylvester) {
+        if (r >= D || c >= D) {
+            v = cmake(0, 0);
+        } else {
+            const cplx* l = reinterpret_cast<const cplx*>(lam);
+            cplx den = adjoint ? cadd(cconj(l[r]), l[c]) : cadd(l[r], cconj(l[c]));
+            den.re -= s_re;
+            den.im -= s_im;
+            const double m = cabs2(den);
+            v = cmake(-(v.re * den.re + v.im * den.im) / m, -(v.im * den.re - v.re * den.im) / m);
+        }
+    }
+    reinterpret_cast<cplx*>(out)[(long long)r * Dp + c] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
 // The connected four-block transfer of a tangent pair (u, w) (blocks 00, ket, bra, ket+bra) applied
 // to nch channels in one launch: output matrix (pair p, channel c, block k) =
 //   jac T(x_c)_k + c jt S(x_{c-1})_k
