@@ -190,6 +190,17 @@ int rcmps_set_nojump_threshold(rcmps_context* ctx, double kappa);
    the eigenvector condition estimate (INFINITY when W is singular). */
 int rcmps_state_nojump_form(const rcmps_state* st, int* schur, double* kappa);
 
+/* The eigendecomposition L = W diag(mu) W^-1 of the state's transfer superoperator
+   L(X) = Q X + X Q^+ + R X R^+ on row-major vec(X) (index i D + j), n = D^2: W and W^-1 column-major
+   n x n, mu n eigenvalues, device pointers (copied).  With it the excitation kinetic resolvents
+   (1 - z (L - ip))^-1 and (1 - z (L* + ip))^-1 of every momentum become products with W and W^-1
+   (the reference's ExcitationContext factors the same transfer densely,
+   src/excitation_environments.jl:18-40) instead of Krylov bases; n = 0 removes it.  Computed on the
+   host by the caller (rcmps_gpu does it for D <= 32).  RCMPS_ERR_ARGUMENT for another n or
+   nonfinite entries. */
+int rcmps_state_set_transfer_eigen(rcmps_context* ctx, rcmps_state* st, int n, const double* W, const double* Winv,
+                                   const double* mu);
+
 /* Tile variant of this context's batched complex products: -1 (default) lets the autotuner pick
    per (padded size, batch class), 0..3 force one (0: one 16x16 tile per four-wave workgroup with
    the k-steps split, 1: four tiles per workgroup, one per wave, 2: one tile per one-wave

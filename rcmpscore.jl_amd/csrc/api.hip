@@ -3,6 +3,7 @@
 // holds the Krylov bookkeeping (Hessenberg columns, Givens rotations) and, once per state, the
 // D x D spectral factorisation of Q that the no-jump preconditioner applies.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <chrono>
@@ -95,6 +96,10 @@ struct TrajBufs {
         E.release(); fixed.release(); flags.release(); out.release();
     }
 };
+
+// square root of a computed squared norm: rounding can leave it slightly negative (clamped to 0), and
+// a NaN stays NaN so that the callers' finiteness checks see it
+inline double norm_sqrt(double v) { return std::isnan(v) ? v : std::sqrt(std::max(0.0, v)); }
 
 inline void ck(hipError_t e) {
     if (e != hipSuccess) throw Fail{RCMPS_ERR_DEVICE};
@@ -215,6 +220,8 @@ struct rcmps_context {
     double nojump_kappa = 1e6;
     // layout of the D x D matrices at the C ABI (RCMPS_LAYOUT_ROW_MAJOR or RCMPS_LAYOUT_COLUMN_MAJOR)
     int layout = RCMPS_LAYOUT_ROW_MAJOR;
+    // rocBLAS handle on this context's stream, for the dense transfer resolvents (created on first use)
+    rocblas_handle blas = nullptr;
 };
 
 struct rcmps_state {
@@ -224,6 +231,12 @@ struct rcmps_state {
     int z2_tiles = 0;
     DevBuf R, K, Q, Qh, Rh, C, Ch, rho, V, Vinv, lam, ident;
     DevBuf VH, VinvH;   // V^+ and V^-+ for the fused two-sided products of the eigenbasis form
+    // eigendecomposition L = W diag(mu) W^-1 of the transfer superoperator on the padded Dp^2 space
+    // (W, W^-1 column-major, identity on the padded indices), set by rcmps_state_set_transfer_eigen:
+    // the excitation kinetic resolvents (1 - z (L - s))^-1 are then products with W and W^-1
+    int tN = 0;
+    int tzero = -1;      // padded index of the zero mode (the stationary state), -1 if not isolated
+    DevBuf tW, tWinv, tmu;
     // no-jump inverse: eigenbasis (V = eigenvectors of Q, Vinv their inverse, lam the eigenvalues) or,
     // when nojump_schur, the Schur basis (V = Z, Vinv = Z^+, Tq = T upper triangular, Q = Z T Z^+)
     int nojump_schur = 0;
@@ -399,7 +412,8 @@ struct Dev {
         double* out = small() + 32;
         hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, x, 0LL, n(), x, out);
         ck(hipGetLastError());
-        return std::sqrt(std::max(0.0, fetch2(out).real()));
+        const double v = fetch2(out).real();
+        return norm_sqrt(v);
     }
 };
 
@@ -624,7 +638,7 @@ bool gmres(Dev& dv, const std::function<void(const double*, double*)>& A, const 
             const cd* hv = reinterpret_cast<const cd*>(slots + (size_t)(k & 1) * col);
             for (int i = 0; i <= k; ++i)
                 H[(size_t)i * m + k] += hv[i] + hv[(m + 1) + i];
-            double hn = std::sqrt(std::max(0.0, hv[2 * (m + 1)].real()));
+            double hn = norm_sqrt(hv[2 * (m + 1)].real());
             H[(size_t)(k + 1) * m + k] = hn;
             for (int i = 0; i < k; ++i) {
                 cd h1 = H[(size_t)i * m + k], h2 = H[(size_t)(i + 1) * m + k];
@@ -849,6 +863,7 @@ void rcmps_context_destroy(rcmps_context* ctx) {
                      &ctx->kry_tmp, &ctx->tune_buf};
     for (auto* b : all) b->release();
     for (auto& b : ctx->tb) b->release();
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     ctx->kin.release();
     for (auto& e : ctx->ev) (void)hipEventDestroy(e);
     for (auto& e : ctx->kry_ev)
@@ -861,7 +876,8 @@ void rcmps_context_destroy(rcmps_context* ctx) {
 void rcmps_state_destroy(rcmps_state* st) {
     if (!st) return;
     DevBuf* all[] = {&st->R, &st->K, &st->Q, &st->Qh, &st->Rh, &st->C, &st->Ch, &st->rho,
-                     &st->V, &st->Vinv, &st->lam, &st->ident, &st->Minv, &st->Tq, &st->VH, &st->VinvH};
+                     &st->V, &st->Vinv, &st->lam, &st->ident, &st->Minv, &st->Tq, &st->VH, &st->VinvH,
+                     &st->tW, &st->tWinv, &st->tmu};
     for (auto* b : all) b->release();
     delete st;
 }
@@ -1643,9 +1659,9 @@ void basis_extend(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasi
     for (int j = k0; j < target; ++j) {
         const cd* col = hv.data() + (size_t)(j - k0) * stride;
         ++kw.actions;
-        const double oldnorm = std::sqrt(std::max(0.0, col[0].real()));
+        const double oldnorm = norm_sqrt(col[0].real());
         for (int i = 0; i <= j; ++i) b.H[(size_t)i * b.cap + j] += col[1 + i] + col[1 + (b.cap + 1) + i];
-        const double beta = std::sqrt(std::max(0.0, col[1 + 2 * (b.cap + 1)].real()));
+        const double beta = norm_sqrt(col[1 + 2 * (b.cap + 1)].real());
         if (!std::isfinite(oldnorm) || !std::isfinite(beta)) throw Fail{RCMPS_ERR_SOLVER};
         b.H[(size_t)(j + 1) * b.cap + j] = beta;
         b.k = j + 1;

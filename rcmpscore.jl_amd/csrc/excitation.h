@@ -69,6 +69,11 @@ struct RcPairArgs {
 };
 
 __global__ void rc_pair_rhs(RcPairArgs g);
+__global__ void rc_res_scale(long long N, int nz, int per_basis, int adjoint, const double* Y, const double* mu,
+                             const double* z, const double* s, double* out);
+__global__ void rc_bcast_rhs(long long N, int nz, const double* src, long long ss, double* out);
+__global__ void rc_poisson_scale(long long N, int Dp, int D, int adjoint, double s, int k0, const double* mu,
+                                 const double* Y, double* out);
 __global__ void rc_lindblad(int Dp, int adjoint, const double* Q, const double* Qh, const double* R, const double* Rh,
                             const double* X, double* out);
 __global__ void rc_two_sided(int Dp, int D, const double* A, const double* X, const double* B, double* out, int sylvester,

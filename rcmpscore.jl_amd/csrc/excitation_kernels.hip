@@ -250,6 +250,52 @@ __global__ void rc_ode_stage(const RcOdeCtl* c, long long N, long long half, dou
     if (active) reinterpret_cast<cplx*>(U1)[i] = cadd(yi, cscale(hp[0], cscale(a10, ki)));
 }
 
+// Resolvent step in the transfer's eigenbasis: out[v][k] = Y[src(v)][k] / (1 - z_i (mu_k + i s_b)) for
+// vector v = b nz + i (src(v) = b when per_basis, else v), mu conjugated for the adjoint; N entries
+// per vector; s null: no shift.  grid (nb nz, ceil(N / 256)).
+__global__ void rc_res_scale(long long N, int nz, int per_basis, int adjoint, const double* Y, const double* mu,
+                             const double* z, const double* s, double* out) {
+    const long long k = (long long)blockIdx.y * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int v = blockIdx.x, b = v / nz, i = v % nz;
+    const cplx y = reinterpret_cast<const cplx*>(Y)[(long long)(per_basis ? b : v) * N + k];
+    cplx l = reinterpret_cast<const cplx*>(mu)[k];
+    if (adjoint) l = cconj(l);
+    const double zi = z[i], sb = s ? s[b] : 0.0;
+    const cplx den = cmake(1.0 - zi * l.re, -zi * (l.im + sb));
+    const double m = cabs2(den);
+    reinterpret_cast<cplx*>(out)[(long long)v * N + k] =
+        cmake((y.re * den.re + y.im * den.im) / m, (y.im * den.re - y.re * den.im) / m);
+}
+
+// The pinned Poisson operator in the transfer's eigenbasis: out_k = Y_k / (-l_k + i s + [k == k0]),
+// l = mu (adjoint: conj mu), k0 the zero mode; zero on the padding of the Dp^2 space (where the
+// eigenvalue is a placeholder 0).  N = Dp^2, grid ceil(N / 256).
+__global__ void rc_poisson_scale(long long N, int Dp, int D, int adjoint, double s, int k0, const double* mu,
+                                 const double* Y, double* out) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    if (k / Dp >= D || k % Dp >= D) {
+        reinterpret_cast<cplx*>(out)[k] = cmake(0, 0);
+        return;
+    }
+    cplx l = reinterpret_cast<const cplx*>(mu)[k];
+    if (adjoint) l = cconj(l);
+    const cplx den = cmake(-l.re + (k == k0 ? 1.0 : 0.0), -l.im + s);
+    const cplx y = reinterpret_cast<const cplx*>(Y)[k];
+    const double m = cabs2(den);
+    reinterpret_cast<cplx*>(out)[k] = cmake((y.re * den.re + y.im * den.im) / m, (y.im * den.re - y.re * den.im) / m);
+}
+
+// out[v] = src[v / nz] (N complex each): block 1 of basis b's right-hand side for each node.
+// grid (nb nz, ceil(N / 256)).
+__global__ void rc_bcast_rhs(long long N, int nz, const double* src, long long ss, double* out) {
+    const long long k = (long long)blockIdx.y * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int v = blockIdx.x;
+    reinterpret_cast<cplx*>(out)[(long long)v * N + k] = reinterpret_cast<const cplx*>(src)[(long long)(v / nz) * ss + k];
+}
+
 // out[b] = tr(A_b B_b) for Dp x Dp matrices; grid (nb)
 __global__ void __launch_bounds__(256) rc_trace_prod_batched(int Dp, const double* A, long long sa, const double* B,
                                                              long long sb, double* out) {

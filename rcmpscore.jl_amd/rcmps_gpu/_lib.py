@@ -106,6 +106,7 @@ SIGNATURES = {
     "rcmps_gemm_variant": (ctypes.c_int, [_P, ctypes.c_int]),
     "rcmps_set_nojump_threshold": (ctypes.c_int, [_P, ctypes.c_double]),
     "rcmps_set_layout": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rcmps_state_set_transfer_eigen": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, _P, _P]),
     "rcmps_state_nojump_form": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
     "rcmps_gemm_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),

@@ -64,13 +64,58 @@ def check_kinetic_nodes(n):
     return int(n)
 
 
+# Largest D for which the excitation kinetic resolvents use the transfer's dense eigendecomposition
+# (one host eigensolve of the D^2 x D^2 transfer per state: about 1.5 s at D = 32, 16 MB per factor).
+DENSE_TRANSFER_MAX_D = 32
+# eigenvector condition estimate above which the dense route is not used (Krylov bases instead)
+DENSE_TRANSFER_MAX_COND = 1e10
+
+
+def transfer_eigen(s: State, enable: bool = True):
+    """Attach (or, with enable=False, remove) the eigendecomposition of the transfer superoperator
+    L(X) = Q X + X Q^+ + R X R^+ to the state on the device, so that the excitation kinetic resolvents
+    of every momentum are products with its eigenvectors (the shifts L - ip and L* + ip keep them).
+    Returns the condition estimate |W|_F |W^-1|_F, or None when the dense route is not used (D above
+    DENSE_TRANSFER_MAX_D, or an ill-conditioned eigenbasis)."""
+    lib = _lib.load()
+    ctx = _lib.context(s.device)
+    if not enable or s.D > DENSE_TRANSFER_MAX_D:
+        _lib.check(lib.rcmps_state_set_transfer_eigen(ctx, s.handle, 0, None, None, None), "transfer eigen")
+        s._transfer_cond = None
+        s._transfer_unusable = enable
+        return None
+    d = s.D
+    eye = np.eye(d)
+    L = np.kron(s.Q, eye) + np.kron(eye, s.Q.conj()) + np.kron(s.R, s.R.conj())
+    mu, W = np.linalg.eig(L)
+    Winv = np.linalg.inv(W)
+    cond = float(np.linalg.norm(W) * np.linalg.norm(Winv))
+    if not (np.isfinite(cond) and cond <= DENSE_TRANSFER_MAX_COND):
+        _lib.check(lib.rcmps_state_set_transfer_eigen(ctx, s.handle, 0, None, None, None), "transfer eigen")
+        s._transfer_cond = None
+        s._transfer_unusable = True
+        return None
+    dev = _device(s)
+    wt = torch.from_numpy(np.ascontiguousarray(W.T)).to(dev)        # column-major W
+    wit = torch.from_numpy(np.ascontiguousarray(Winv.T)).to(dev)
+    mut = torch.from_numpy(np.ascontiguousarray(mu)).to(dev)
+    torch.cuda.synchronize(dev)
+    _lib.check(lib.rcmps_state_set_transfer_eigen(ctx, s.handle, d * d, wt.data_ptr(), wit.data_ptr(), mut.data_ptr()),
+               "transfer eigen")
+    s._transfer_cond = cond
+    s._transfer_unusable = False
+    return cond
+
+
 class ExcitationContext:
     """ExcitationContext(s, p; tol) (src/excitation_environments.jl:18-40) on the device.  Built
     matrix-free: the mixing check of the dense reference is replaced by the residual checks of the
     Poisson and resolvent solves.  `integrator` overrides the state's integration settings for the
-    potential term (the refined settings of the validation)."""
+    potential term (the refined settings of the validation).  dense_transfer: for D <= 32 the state's
+    transfer eigendecomposition (transfer_eigen, once per state) serves the kinetic resolvents;
+    False keeps the Krylov bases (and removes a decomposition the state carries)."""
 
-    def __init__(self, s: State, p: float = 0.0, *, tol: float = 1e-10, integrator=None):
+    def __init__(self, s: State, p: float = 0.0, *, tol: float = 1e-10, integrator=None, dense_transfer: bool = True):
         if not (math.isfinite(p) and math.isfinite(tol) and tol > 0):
             raise ValueError("invalid momentum or environment tolerance")
         self.state = s
@@ -78,6 +123,11 @@ class ExcitationContext:
         self.tol = float(tol)
         self.integrator = copy.deepcopy(integrator or s.integrator_param)
         lib = _lib.load()
+        if not dense_transfer:
+            if getattr(s, "_transfer_cond", None) is not None:
+                transfer_eigen(s, False)
+        elif getattr(s, "_transfer_cond", None) is None and not getattr(s, "_transfer_unusable", False):
+            transfer_eigen(s, True)
         handle = ctypes.c_void_p()
         _lib.check(lib.rcmps_excitation_create(_lib.context(s.device), s.handle, self.momentum, self.tol,
                                                ctypes.byref(handle)), "excitation context")

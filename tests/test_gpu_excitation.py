@@ -15,6 +15,7 @@ energy and the acceptance, within their tolerance 1e-11.
 """
 import json
 import math
+import time
 import os
 import sys
 from pathlib import Path
@@ -54,15 +55,17 @@ def oracle_state(s, ip):
     return X.XState(s.R, s.Q, s.rho, n_quad=ip.n_quad, max_point=ip.max_point, atol=ip.atol, rtol=ip.rtol)
 
 
+@pytest.mark.parametrize("dense", [True, False], ids=["dense_transfer", "krylov"])
 @pytest.mark.parametrize("d,seed,p", [(2, 42, 0.0), (2, 42, 0.7), (4, 7, 0.7)])
-def test_action_parts_match_oracle(rg, d, seed, p):
-    note(f"case: excitation action D={d} seed={seed} p={p}")
+def test_action_parts_match_oracle(rg, d, seed, p, dense):
+    note(f"case: excitation action D={d} seed={seed} p={p} ({'dense transfer' if dense else 'Krylov'} resolvents)")
     from rcmps_gpu import IntegratorParameters
     from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
     R, K = random_arrays(d, seed, 0.5)
     s = rg.State(R, K, steady_tol=1e-13)
     s.integrator_param = IntegratorParameters(n_quad=64, max_point=18.0, atol=1e-12, rtol=1e-12)
-    ctx = ExcitationContext(s, p)
+    ctx = ExcitationContext(s, p, dense_transfer=dense)
+    assert (getattr(s, "_transfer_cond", None) is not None) == dense
     rng = np.random.default_rng(seed + 1)
     w = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
     model = ExcitationModel("phi4", 1.0)
@@ -80,6 +83,35 @@ def test_action_parts_match_oracle(rg, d, seed, p):
     assert ep < 1e-9
     full = rg.excitation_hamiltonian_action(ctx, w, model=model, kinetic_nodes=32)
     assert np.abs(full - (kin + pot)).max() <= 1e-12 * np.abs(full).max()
+
+
+@pytest.mark.parametrize("name,D", [("phi4_g2_D16_z2_state", 16), ("phi4_g2.7_D32_z2_state", 32)])
+def test_dense_transfer_resolvents_match_krylov(rg, name, D):
+    """The kinetic part of F(W) for a batch of three tangents at two momenta on the reference's Z2
+    checkpoints: resolvents from the transfer's eigendecomposition against the Krylov bases (solved
+    to 1e-11), within 1e-8 of |F|."""
+    note(f"case: dense transfer vs Krylov resolvents, {name}")
+    from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
+    from rcmps_gpu.jld2 import read_jld2
+    rec = read_jld2(GOLDEN / f"{name}.jld2")
+    s = rg.State(rec.R, rec.K, steady_tol=1e-12)
+    model = ExcitationModel("phi4", rec.coupling)
+    rng = np.random.default_rng(5)
+    ws = [rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D)) for _ in range(3)]
+    tight = rg.KineticKrylovOptions(tolerance=1e-11, fallback_tolerance=1e-11)
+    out = {}
+    for dense in (True, False):
+        ctxs = [ExcitationContext(s, p, dense_transfer=dense) for p in (0.1, 0.1, 0.6)]
+        t0 = time.perf_counter()
+        fs = action_batch(ctxs, ws, model, 64, parts=1, krylov_options=None if dense else tight)
+        out[dense] = (fs, time.perf_counter() - t0)
+        for c in ctxs:
+            c.close()
+    note(f"  kinetic part of 3 tangents: dense {out[True][1]:.3f} s, Krylov {out[False][1]:.3f} s, "
+         f"condition {getattr(s, '_transfer_cond', None)}")
+    for a, b in zip(out[True][0], out[False][0]):
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+    s.close()
 
 
 @pytest.mark.parametrize("d,seed,p", [(2, 42, 0.7), (4, 7, 0.0)])
@@ -140,12 +172,14 @@ def test_matrix_element_hermitian_and_adaptive(rg):
     assert abs(uw - ref) <= 1e-9 * scale
 
 
-def test_poisson_solves_match_oracle(rg):
-    note("case: excitation Poisson solves D=4 p=0.7")
+@pytest.mark.parametrize("dense", [True, False], ids=["dense_transfer", "gmres"])
+def test_poisson_solves_match_oracle(rg, dense):
+    """The pinned Poisson solves, by the transfer's eigenbasis (dense_transfer) or by GMRES."""
+    note(f"case: excitation Poisson solves D=4 p=0.7 ({'dense transfer' if dense else 'GMRES'})")
     from rcmps_gpu.excitation import ExcitationContext, action_dual_poisson, excitation_poisson
     R, K = random_arrays(4, 3, 0.5)
     s = rg.State(R, K, steady_tol=1e-13)
-    ctx = ExcitationContext(s, 0.7)
+    ctx = ExcitationContext(s, 0.7, dense_transfer=dense)
     octx = X.context(oracle_state(s, s.integrator_param), 0.7)
     rng = np.random.default_rng(9)
     b = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))

@@ -136,7 +136,7 @@ def test_excitation_poisson_in_the_schur_form(rg):
     finally:
         rg.nojump_threshold(1e6)
     assert s.nojump_form[0] == "schur"
-    ctx = ExcitationContext(s, 0.7)
+    ctx = ExcitationContext(s, 0.7, dense_transfer=False)   # the GMRES route, preconditioned in the Schur form
     ip = s.integrator_param
     octx = X.context(X.XState(s.R, s.Q, s.rho, n_quad=ip.n_quad, max_point=ip.max_point, atol=ip.atol,
                               rtol=ip.rtol), 0.7)
