@@ -75,6 +75,10 @@ typedef struct rcmps_excitation rcmps_excitation;
 #define RCMPS_EXC_DENSE 4         /* with either: ignore a Z2 state's parity blocks in the action ODE
                                      (the same bits as the default, which skips the exactly zero ones
                                      when every tangent has one parity and D = 32, 64) */
+#define RCMPS_EXC_GRAPH 8         /* with either: step the action ODE by replayed launch graphs */
+#define RCMPS_EXC_PERSISTENT 16   /* with either: integrate it in one persistent launch wherever every
+                                     tile's workgroup fits on the device at once (neither flag: the
+                                     faster of the two for the size; all three give the same bits) */
 
 /* IntegratorParameters (src/state.jl:4-10) plus the DOP853 initial step. */
 typedef struct {
@@ -107,7 +111,9 @@ typedef struct {
        inputs, load inputs, level-1 products, level-2 products before the neighbour tiles, wait for
        neighbour tiles, load them, remaining products and epilogue, integrator bookkeeping,
        step end (error norm, record, all-gather), whole kernel, publishing the next stage input,
-       stage entry */
+       stage entry.  The excitation action: [0][0..3] its kinetic phases; [4][0..3] workgroup 0 of
+       the persistent action ODE launch in the right-hand-side products, the exchanges between
+       stages, the step controller and the whole launch */
     double phase_us[6][12];
     /* infinite-line kinetic backend: angular nodes of the accepted level, basis sizes per
        (op R/[Q,R]) x (forward/adjoint), operator applications, largest true relative residual */

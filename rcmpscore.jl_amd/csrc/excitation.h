@@ -1,6 +1,7 @@
 // Host-side declarations of the excitation kernels (excitation_kernels.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 // coefficients of one DOP853 combination (up to 13 stage vectors)
 struct RcStageCoef {
@@ -101,6 +102,33 @@ struct RcOdeCtl {
     int si, node, nstops, npts;
     int done, active, commit, sample, last_rejected, clipped, status, pad;
 };
+// the persistent integration (rc_quartic_ode): operands, exchange buffers and the controller at entry
+struct RcOdeRun {
+    RcQuarticArgs g;           // the right-hand side's operands (U, F, epilogue fields unused)
+    double* ybuf;              // [N] the state at entry, the end state on exit
+    double* u0;                // [N] stage inputs exchanged between workgroups (two buffers)
+    double* u1;
+    double* slots;             // [nwg][2] per-tile error sums
+    uint32_t* flags;           // [nwg] per-workgroup event counters, zero at entry
+    double* Sf;                // samples, as OdeWork
+    double* Sa;
+    long long N, half;
+    int nwg, ntrack, nranges, pad;
+    const int* ranges;
+    const double* stops;
+    const int* stop_node;
+    const double* cheb;
+    RcOdeCtl c0;               // the controller at entry (the first attempt is set up in the kernel)
+    double jac0, jt0, atol, rtol, cnt;
+    long long wait_ticks;      // budget of one wait (constant-rate clock ticks)
+    uint32_t* abort_flag;
+    double* out;               // [8] status, attempts, accepted, rejected; workgroup 0's clock ticks in
+                               // the products, the exchanges, the controller and the whole launch
+};
+__global__ void rc_quartic_ode(const RcOdeRun* pp);
+#define RC_QUARTIC_ODE_MAX_WG 256   // workgroups one exchange round of rc_quartic_ode polls
+hipError_t rc_quartic_ode_capacity(int* blocks);
+hipError_t rc_launch_quartic_ode(dim3 grid, const RcOdeRun* dparams, hipStream_t stream);
 __global__ void rc_ode_step(RcOdeCtl* c, int finish, const double* partial, const int* ranges, int nranges,
                             int ntrack, double cnt, const double* stops, const int* stop_node, const double* cheb,
                             double* cf);

@@ -111,10 +111,12 @@ __global__ void rc_stage(long long N, const double* y, const double* K, long lon
 // worst over trajectories; accept or reject and the next step size.  Then, unless done, set up the
 // next attempt: the step clipped at the next stop (a quadrature node, 0 or the window's end) and the
 // source coefficients [2|t_s|, s(t_s)] of stages 1..12 (one thread each) with the step at cf[26].
-__global__ void __launch_bounds__(256) rc_ode_step(RcOdeCtl* c, int finish, const double* partial,
-                                                   const int* ranges, int nranges, int ntrack, double cnt,
-                                                   const double* stops, const int* stop_node, const double* cheb,
-                                                   double* cf) {
+// The controller step shared by rc_ode_step and the persistent launch (c in device or LDS memory; C:
+// the error partials are read with device-coherent loads).  No barrier after the cf writes.
+template <bool C>
+__device__ __forceinline__ void ode_step_body(RcOdeCtl* c, int finish, const double* partial, const int* ranges,
+                                              int nranges, int ntrack, double cnt, const double* stops,
+                                              const int* stop_node, const double* cheb, double* cf) {
     __shared__ double smax[4];
     __shared__ int sbad[4];
     __shared__ double s_t, s_he, s_tn;
@@ -130,8 +132,14 @@ __global__ void __launch_bounds__(256) rc_ode_step(RcOdeCtl* c, int finish, cons
             for (int r = 0; r < nranges; ++r) {
                 const int first = ranges[2 * (tr * nranges + r)], count = ranges[2 * (tr * nranges + r) + 1];
                 for (int q = lane; q < count; q += 64) {
-                    e5 += partial[2 * (first + q)];
-                    e3 += partial[2 * (first + q) + 1];
+                    if constexpr (C) {
+                        e5 += __hip_atomic_load(partial + 2 * (first + q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        e3 += __hip_atomic_load(partial + 2 * (first + q) + 1, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+                    } else {
+                        e5 += partial[2 * (first + q)];
+                        e3 += partial[2 * (first + q) + 1];
+                    }
                 }
             }
 #pragma unroll
@@ -220,6 +228,13 @@ __global__ void __launch_bounds__(256) rc_ode_step(RcOdeCtl* c, int finish, cons
         cf[2 * tid + 1] = rc_source(ts, cheb);
     }
     if (s_go && tid == 0) cf[26] = s_he;
+}
+
+__global__ void __launch_bounds__(256) rc_ode_step(RcOdeCtl* c, int finish, const double* partial,
+                                                   const int* ranges, int nranges, int ntrack, double cnt,
+                                                   const double* stops, const int* stop_node, const double* cheb,
+                                                   double* cf) {
+    ode_step_body<false>(c, finish, partial, ranges, nranges, ntrack, cnt, stops, stop_node, cheb, cf);
 }
 
 // First stage input of the attempt, U1 = y + h a10 K_0, preceded by the commit of the previous
@@ -344,16 +359,35 @@ namespace {
 // Every fragment of this wave's k-steps (Dp / 16 of them, at most 4) is loaded before the first MFMA,
 // so a product pays one load round trip instead of one per k-step; the MFMAs run in the same k order.
 // Wave w runs k-step w + 4 i of k-tile i.  kmask (uniform) selects the k-tiles: bit i for k-tile i.
+// The stage input as a buffer resource (device-coherent 16-byte loads of the elements another
+// workgroup of the same launch wrote; offsets outside it read zero).
+struct QCoh {
+    __amdgpu_buffer_rsrc_t r;
+    const cplx* base;
+};
+
+// An operand element; C: a device-coherent load from the stage input q.
+template <bool C>
+__device__ __forceinline__ cplx qld(const QCoh* q, const cplx* p) {
+    if constexpr (C) {
+        return ld_coh(q->r, (uint32_t)((p - q->base) * (long long)sizeof(cplx)));
+    } else {
+        (void)q;
+        return *p;
+    }
+}
+
+template <bool CA = false, bool CB = false>
 __device__ __forceinline__ void qprod(cacc& acc, const cplx* A, const cplx* B, int Dp, int bi, int bj, int wave,
-                                      int lane, double scale, int kmask = 15) {
+                                      int lane, double scale, int kmask = 15, const QCoh* q = nullptr) {
     const int nk = Dp / 16;
     cplx a[4], b[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
         if (i < nk && ((kmask >> i) & 1)) {
             const int kk = 4 * (wave + 4 * i) + (lane >> 4);
-            a[i] = A[(16 * bi + (lane & 15)) * Dp + kk];
-            b[i] = B[kk * Dp + 16 * bj + (lane & 15)];
+            a[i] = qld<CA>(q, &A[(16 * bi + (lane & 15)) * Dp + kk]);
+            b[i] = qld<CB>(q, &B[kk * Dp + 16 * bj + (lane & 15)]);
         }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -401,8 +435,9 @@ __device__ __forceinline__ cplx qreduce(const cacc& acc, cplx* red, int wave, in
 
 // One wave's whole product A(bi, :) B(:, bj) over every k-step (no cross-wave reduction), one k-tile of
 // fragments loaded ahead of its four MFMAs; kmask selects the k-tiles (bit t: k-steps 4t .. 4t + 3).
+template <bool CA = false>
 __device__ __forceinline__ void qprod_rows(cacc& acc, const cplx* A, const cplx* B, int Dp, int bi, int bj, int lane,
-                                           int kmask) {
+                                           int kmask, const QCoh* q = nullptr) {
     const int nk = Dp / 16;
     for (int t = 0; t < nk; ++t) {
         if (!((kmask >> t) & 1)) continue;
@@ -410,7 +445,7 @@ __device__ __forceinline__ void qprod_rows(cacc& acc, const cplx* A, const cplx*
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int kk = 16 * t + 4 * s + (lane >> 4);
-            a[s] = A[(16 * bi + (lane & 15)) * Dp + kk];
+            a[s] = qld<CA>(q, &A[(16 * bi + (lane & 15)) * Dp + kk]);
             b[s] = B[kk * Dp + 16 * bj + (lane & 15)];
         }
 #pragma unroll
@@ -428,8 +463,9 @@ struct QTerm {
 // Column blocks (Dp x 16, in LDS) T0 = a0 + a1 and, when T1 is given, T1 = b0 + b1.  The row tiles of
 // both blocks are spread over the waves, each wave running every k-step of its row tiles and writing
 // them straight to LDS; one barrier at the end.  zt: Z2 parity blocks (0: every k-tile).
+template <bool CX = false>
 __device__ __forceinline__ void qcolumns(cplx* T0, QTerm a0, QTerm a1, cplx* T1, QTerm b0, QTerm b1, int Dp, int bj,
-                                         int wave, int lane, int zt) {
+                                         int wave, int lane, int zt, const QCoh* qc = nullptr) {
     const int nbt = Dp / 16, ntask = (T1 ? 2 : 1) * nbt;
     for (int i = wave; i < ntask; i += 4) {
         const int col = i / nbt, rt = i % nbt;
@@ -437,8 +473,8 @@ __device__ __forceinline__ void qcolumns(cplx* T0, QTerm a0, QTerm a1, cplx* T1,
         cplx* T = col ? T1 : T0;
         cacc acc;
         cacc_zero(acc);
-        qprod_rows(acc, p.X, p.B, Dp, rt, bj, lane, zmask(zt, nbt, rt, bj, p.pa, p.pb));
-        if (q.X) qprod_rows(acc, q.X, q.B, Dp, rt, bj, lane, zmask(zt, nbt, rt, bj, q.pa, q.pb));
+        qprod_rows<CX>(acc, p.X, p.B, Dp, rt, bj, lane, zmask(zt, nbt, rt, bj, p.pa, p.pb), qc);
+        if (q.X) qprod_rows<CX>(acc, q.X, q.B, Dp, rt, bj, lane, zmask(zt, nbt, rt, bj, q.pa, q.pb), qc);
 #pragma unroll
         for (int e = 0; e < 4; ++e) T[(16 * rt + 4 * e + (lane >> 4)) * 16 + (lane & 15)] = cacc_get(acc, e);
     }
@@ -447,16 +483,17 @@ __device__ __forceinline__ void qcolumns(cplx* T0, QTerm a0, QTerm a1, cplx* T1,
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
-    __shared__ cplx red[1024];
-    __shared__ cplx T1[64 * 16];
-    __shared__ cplx T2[64 * 16];
-    if (g.active && !*g.active) return;
+// One right-hand-side tile of the :phi^4: action ODE (the products of rc_quartic_rhs for output tile
+// (bi, bj) of matrix mi = blockIdx.y, reduced over the waves): thread t receives element (t/16, t%16).
+// U: the stage input; C: its elements are read with device-coherent loads (another workgroup of the
+// same launch wrote them).
+template <bool C>
+__device__ __forceinline__ cplx quartic_tile(const RcQuarticArgs& g, const cplx* U, double jac, double jt, int bi,
+                                             int bj, int mi, cplx* red, cplx* T1, cplx* T2) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int Dp = g.Dp, nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
-    const int nb = g.na * 5, mi = blockIdx.y, blk = mi / nb, r = mi % nb, a = r / 5, k = r % 5;
+    const int Dp = g.Dp;
+    const int nb = g.na * 5, blk = mi / nb, r = mi % nb, a = r / 5, k = r % 5;
     const long long m = (long long)Dp * Dp;
-    const cplx* U = reinterpret_cast<const cplx*>(g.U);
     auto mat = [&](int b, int ch) { return U + ((long long)b * nb + a * 5 + ch) * m; };
     const cplx* Q = reinterpret_cast<const cplx*>(g.Q);
     const cplx* Qh = reinterpret_cast<const cplx*>(g.Qh);
@@ -466,81 +503,96 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
     const cplx* Rh = reinterpret_cast<const cplx*>(g.Rh);
     const cplx* v = reinterpret_cast<const cplx*>(g.Vrep) + (long long)a * 5 * m;
     const cplx* w = reinterpret_cast<const cplx*>(g.Wrep) + (long long)a * 5 * m;
-    const double jac = g.coef ? g.coef[0] : g.jac;
-    const double jt = g.coef ? g.coef[1] : g.jt;
     // Z2 parities (g.zt > 0): Q, Q^+ and the shifted ones even, R, R^+ odd, w = W of parity pw,
     // v = -R^+ W of parity pw ^ 1; channels x_k and a1_k of parity k & 1 (x_0 = rho, a1 of the last
     // channel = I), y_k and a0_k of parity (k + 1 + pw) & 1 (y_0 = R W, a0 driven by a1 w and a1 v)
     const int zt = g.zt, nk = Dp / 16, pw = g.wpar;
     const int px = k & 1, py = (k + 1 + pw) & 1;
     auto zm = [&](int pa, int pb) { return zmask(zt, nk, bi, bj, pa, pb); };
+    const QCoh qc{rc_rsrc(U, (uint32_t)(4LL * nb * m * (long long)sizeof(cplx))), U};
     cacc acc;
     cacc_zero(acc);
     if (blk == 0) {
         const cplx* x = mat(0, k);
         if (jac != 0.0) {
-            qprod(acc, Q, x, Dp, bi, bj, wave, lane, jac, zm(0, px));
-            qprod(acc, x, Qh, Dp, bi, bj, wave, lane, jac, zm(px, 0));
-            qcolumns(T1, QTerm{x, Rh, px, 1}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt);
+            qprod<false, C>(acc, Q, x, Dp, bi, bj, wave, lane, jac, zm(0, px), &qc);
+            qprod<C, false>(acc, x, Qh, Dp, bi, bj, wave, lane, jac, zm(px, 0), &qc);
+            qcolumns<C>(T1, QTerm{x, Rh, px, 1}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt, &qc);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac, zm(1, px ^ 1));
         }
         if (k >= 1) {
             const cplx* xp = mat(0, k - 1);
-            qprod(acc, R, xp, Dp, bi, bj, wave, lane, k * jt, zm(1, px ^ 1));
-            qprod(acc, xp, Rh, Dp, bi, bj, wave, lane, k * jt, zm(px ^ 1, 1));
+            qprod<false, C>(acc, R, xp, Dp, bi, bj, wave, lane, k * jt, zm(1, px ^ 1), &qc);
+            qprod<C, false>(acc, xp, Rh, Dp, bi, bj, wave, lane, k * jt, zm(px ^ 1, 1), &qc);
         }
     } else if (blk == 1) {
         const cplx* y = mat(1, k);
         const cplx* x = mat(0, k);
         if (jac != 0.0) {
-            qprod(acc, Qs, y, Dp, bi, bj, wave, lane, jac, zm(0, py));
-            qprod(acc, y, Qh, Dp, bi, bj, wave, lane, jac, zm(py, 0));
-            qprod(acc, v, x, Dp, bi, bj, wave, lane, jac, zm(pw ^ 1, px));
-            qcolumns(T1, QTerm{y, Rh, py, 1}, QTerm{}, T2, QTerm{x, Rh, px, 1}, QTerm{}, Dp, bj, wave, lane, zt);
+            qprod<false, C>(acc, Qs, y, Dp, bi, bj, wave, lane, jac, zm(0, py), &qc);
+            qprod<C, false>(acc, y, Qh, Dp, bi, bj, wave, lane, jac, zm(py, 0), &qc);
+            qprod<false, C>(acc, v, x, Dp, bi, bj, wave, lane, jac, zm(pw ^ 1, px), &qc);
+            qcolumns<C>(T1, QTerm{y, Rh, py, 1}, QTerm{}, T2, QTerm{x, Rh, px, 1}, QTerm{}, Dp, bj, wave, lane, zt, &qc);
             qprod_t(acc, R, T1, Dp, bi, wave, lane, jac, zm(1, py ^ 1));
             qprod_t(acc, w, T2, Dp, bi, wave, lane, jac, zm(pw, px ^ 1));
         }
         if (k >= 1) {
             const cplx* yp = mat(1, k - 1);
             const cplx* xp = mat(0, k - 1);
-            qprod(acc, R, yp, Dp, bi, bj, wave, lane, k * jt, zm(1, py ^ 1));
-            qprod(acc, yp, Rh, Dp, bi, bj, wave, lane, k * jt, zm(py ^ 1, 1));
-            qprod(acc, w, xp, Dp, bi, bj, wave, lane, k * jt, zm(pw, px ^ 1));
+            qprod<false, C>(acc, R, yp, Dp, bi, bj, wave, lane, k * jt, zm(1, py ^ 1), &qc);
+            qprod<C, false>(acc, yp, Rh, Dp, bi, bj, wave, lane, k * jt, zm(py ^ 1, 1), &qc);
+            qprod<false, C>(acc, w, xp, Dp, bi, bj, wave, lane, k * jt, zm(pw, px ^ 1), &qc);
         }
     } else if (blk == 2) {
         const cplx* a0 = mat(2, k);
         const cplx* a1 = mat(3, k);
         if (jac != 0.0) {
-            qprod(acc, Qhs, a0, Dp, bi, bj, wave, lane, jac, zm(0, py));
-            qprod(acc, a0, Q, Dp, bi, bj, wave, lane, jac, zm(py, 0));
-            qprod(acc, a1, v, Dp, bi, bj, wave, lane, jac, zm(px, pw ^ 1));
-            qcolumns(T1, QTerm{a0, R, py, 1}, QTerm{a1, w, px, pw}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt);
+            qprod<false, C>(acc, Qhs, a0, Dp, bi, bj, wave, lane, jac, zm(0, py), &qc);
+            qprod<C, false>(acc, a0, Q, Dp, bi, bj, wave, lane, jac, zm(py, 0), &qc);
+            qprod<C, false>(acc, a1, v, Dp, bi, bj, wave, lane, jac, zm(px, pw ^ 1), &qc);
+            qcolumns<C>(T1, QTerm{a0, R, py, 1}, QTerm{a1, w, px, pw}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt, &qc);
             qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac, zm(1, py ^ 1));
         }
         if (k <= 3) {
             const double c = (k + 1) * jt;
             const cplx* a0n = mat(2, k + 1);
             const cplx* a1n = mat(3, k + 1);
-            qprod(acc, a0n, R, Dp, bi, bj, wave, lane, c, zm(py ^ 1, 1));
-            qprod(acc, a1n, w, Dp, bi, bj, wave, lane, c, zm(px ^ 1, pw));
-            qprod(acc, Rh, a0n, Dp, bi, bj, wave, lane, c, zm(1, py ^ 1));
+            qprod<C, false>(acc, a0n, R, Dp, bi, bj, wave, lane, c, zm(py ^ 1, 1), &qc);
+            qprod<C, false>(acc, a1n, w, Dp, bi, bj, wave, lane, c, zm(px ^ 1, pw), &qc);
+            qprod<false, C>(acc, Rh, a0n, Dp, bi, bj, wave, lane, c, zm(1, py ^ 1), &qc);
         }
     } else {
         const cplx* a1 = mat(3, k);
         if (jac != 0.0) {
-            qprod(acc, Qh, a1, Dp, bi, bj, wave, lane, jac, zm(0, px));
-            qprod(acc, a1, Q, Dp, bi, bj, wave, lane, jac, zm(px, 0));
-            qcolumns(T1, QTerm{a1, R, px, 1}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt);
+            qprod<false, C>(acc, Qh, a1, Dp, bi, bj, wave, lane, jac, zm(0, px), &qc);
+            qprod<C, false>(acc, a1, Q, Dp, bi, bj, wave, lane, jac, zm(px, 0), &qc);
+            qcolumns<C>(T1, QTerm{a1, R, px, 1}, QTerm{}, nullptr, QTerm{}, QTerm{}, Dp, bj, wave, lane, zt, &qc);
             qprod_t(acc, Rh, T1, Dp, bi, wave, lane, jac, zm(1, px ^ 1));
         }
         if (k <= 3) {
             const double c = (k + 1) * jt;
             const cplx* a1n = mat(3, k + 1);
-            qprod(acc, a1n, R, Dp, bi, bj, wave, lane, c, zm(px ^ 1, 1));
-            qprod(acc, Rh, a1n, Dp, bi, bj, wave, lane, c, zm(1, px ^ 1));
+            qprod<C, false>(acc, a1n, R, Dp, bi, bj, wave, lane, c, zm(px ^ 1, 1), &qc);
+            qprod<false, C>(acc, Rh, a1n, Dp, bi, bj, wave, lane, c, zm(1, px ^ 1), &qc);
         }
     }
-    const cplx val = qreduce(acc, red, wave, lane);
+    (void)tid;
+    return qreduce(acc, red, wave, lane);
+}
+
+__global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T1[64 * 16];
+    __shared__ cplx T2[64 * 16];
+    if (g.active && !*g.active) return;
+    const int tid = threadIdx.x;
+    const int Dp = g.Dp, nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
+    const int mi = blockIdx.y;
+    const long long m = (long long)Dp * Dp;
+    const cplx* U = reinterpret_cast<const cplx*>(g.U);
+    const double jac = g.coef ? g.coef[0] : g.jac;
+    const double jt = g.coef ? g.coef[1] : g.jt;
+    const cplx val = quartic_tile<false>(g, U, jac, jt, bi, bj, mi, red, T1, T2);
     const long long e = (long long)mi * m + (16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15);
     reinterpret_cast<cplx*>(g.F)[e] = val;
     if (g.epi == 0) return;
@@ -575,6 +627,184 @@ __global__ void __launch_bounds__(256) rc_quartic_rhs(RcQuarticArgs g) {
         g.partial[2 * slot] = s5[0];
         g.partial[2 * slot + 1] = s3[0];
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The whole :phi^4: action ODE integration in one persistent launch (cooperative: every workgroup
+// resident).  Workgroup (bi, bj, mi) owns the same 16 x 16 tile as in rc_quartic_rhs and keeps its
+// elements of y, of the stage derivatives K_0..K_12 (LDS) and of the step end in registers.  Each
+// stage input is exchanged through `ubuf` (two buffers, device-coherent stores and loads) with one
+// all-workgroup event per stage: every workgroup publishes its counter and waits for all others.  The
+// step controller runs redundantly in every workgroup on the same error partials (exchanged the same
+// way), so every workgroup takes the same decisions; the arithmetic is the graph path's, so the
+// results are the same bits.  Waits are bounded in wall time: a workgroup that outlasts the budget
+// sets the abort word and every workgroup leaves with RC_ST_WAIT_LIMIT.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rc_quartic_ode(const RcOdeRun* __restrict__ pp) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T1[64 * 16];
+    __shared__ cplx T2[64 * 16];
+    __shared__ cplx sK[13][256];
+    __shared__ RcOdeCtl sc;
+    __shared__ double s_cf[32];
+    __shared__ int s_abort;
+    const RcOdeRun& p = *pp;
+    const RcQuarticArgs& g = p.g;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int Dp = g.Dp, nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt, mi = blockIdx.y;
+    const int wg = mi * nbt * nbt + (int)blockIdx.x;   // also the tile's error slot
+    const long long m = (long long)Dp * Dp;
+    const long long e = (long long)mi * m + (16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15);
+    cplx* const u0 = reinterpret_cast<cplx*>(p.u0);
+    cplx* const u1 = reinterpret_cast<cplx*>(p.u1);
+    uint32_t ev = 0;
+    auto put = [&](int buf, cplx v) {
+        double* d = reinterpret_cast<double*>((buf ? u1 : u0) + e);
+        __hip_atomic_store(d, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(d + 1, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // publish this workgroup's next event once its stores are complete, then wait until every
+    // workgroup has published it (wave 0 polls every counter in each round: lane l those of
+    // workgroups l, l + 64, l + 128, l + 192; at most 256 workgroups)
+    auto sync_all = [&]() -> bool {
+        ++ev;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) st_flag(p.flags + wg, ev);
+        if (tid < 64) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            int bad = 0;
+            for (uint32_t polls = 1;; ++polls) {
+                uint32_t fv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) fv[q] = lane + 64 * q < p.nwg ? ld_flag(p.flags + lane + 64 * q) : ev;
+                const uint32_t ab = ld_flag(p.abort_flag);
+                if (__all(fv[0] >= ev && fv[1] >= ev && fv[2] >= ev && fv[3] >= ev)) break;
+                if (ab) { bad = 1; break; }
+                if ((polls & 63) == 0 && (long long)(__builtin_amdgcn_s_memrealtime() - t0) > p.wait_ticks) {
+                    bad = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (lane == 0) {
+                if (bad) st_flag(p.abort_flag, 1u);
+                s_abort = bad;
+            }
+        }
+        __syncthreads();
+        return s_abort == 0;
+    };
+    // workgroup 0's time (constant-rate clock) in the products, the exchanges and the controller
+    uint64_t t_rhs = 0, t_sync = 0, t_ctl = 0, tk = __builtin_amdgcn_s_memrealtime();
+    const uint64_t t_start = tk;
+    auto lap = [&](uint64_t& acc) {
+        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        acc += now - tk;
+        tk = now;
+    };
+    cplx yv = reinterpret_cast<const cplx*>(p.ybuf)[e];
+    if (tid == 0) sc = p.c0;
+    bool ok = true;
+    // the first right-hand side (first same as last) on y, exchanged through buffer 1 (the first
+    // attempt writes buffer 0 before its event, buffer 1 only after the next)
+    put(1, yv);
+    if (!(ok = sync_all())) goto done;
+    sK[0][tid] = quartic_tile<true>(g, u1, p.jac0, p.jt0, bi, bj, mi, red, T1, T2);
+    ode_step_body<true>(&sc, 0, p.slots, p.ranges, p.nranges, p.ntrack, p.cnt, p.stops, p.stop_node, p.cheb, s_cf);
+    __syncthreads();
+    while (sc.active) {
+        const double he = s_cf[26];
+        put(0, cadd(yv, cscale(he, cscale(RC_A[1][0], sK[0][tid]))));
+        lap(t_ctl);
+        if (!(ok = sync_all())) break;
+        lap(t_sync);
+        cplx ynew = cmake(0, 0);
+        for (int st = 1; st < 13; ++st) {
+            const cplx F = quartic_tile<true>(g, ((st - 1) & 1) ? u1 : u0, s_cf[2 * st], s_cf[2 * st + 1], bi, bj,
+                                              mi, red, T1, T2);
+            lap(t_rhs);
+            sK[st][tid] = F;
+            if (st < 12) {   // the next stage input (stage 12's is the step end, B weights)
+                cplx sum = cscale(st < 11 ? RC_A[st + 1][st] : RC_B[st], F);
+                for (int j = 0; j < st; ++j) {
+                    const double aj = st < 11 ? RC_A[st + 1][j] : RC_B[j];
+                    if (aj != 0.0) sum = cadd(sum, cscale(aj, sK[j][tid]));
+                }
+                const cplx u = cadd(yv, cscale(he, sum));
+                if (st == 11) ynew = u;
+                put(st & 1, u);
+                if (!(ok = sync_all())) break;
+                lap(t_sync);
+            } else {         // error sums of this tile
+                cplx r5 = cscale(RC_E5[12], F), r3 = cscale(RC_E3[12], F);
+                for (int j = 0; j < 12; ++j) {
+                    const cplx kj = sK[j][tid];
+                    if (RC_E5[j] != 0.0) r5 = cadd(r5, cscale(RC_E5[j], kj));
+                    if (RC_E3[j] != 0.0) r3 = cadd(r3, cscale(RC_E3[j], kj));
+                }
+                const double scl = p.atol + p.rtol * fmax(cabsd(yv), cabsd(ynew));
+                double* s5 = reinterpret_cast<double*>(red);
+                double* s3 = s5 + 256;
+                s5[tid] = cabs2(r5) / (scl * scl);
+                s3[tid] = cabs2(r3) / (scl * scl);
+                __syncthreads();
+                for (int hh = 128; hh > 0; hh >>= 1) {
+                    if (tid < hh) { s5[tid] += s5[tid + hh]; s3[tid] += s3[tid + hh]; }
+                    __syncthreads();
+                }
+                if (tid == 0) {
+                    __hip_atomic_store(p.slots + 2 * wg, s5[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(p.slots + 2 * wg + 1, s3[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (!(ok = sync_all())) break;
+                lap(t_sync);
+            }
+        }
+        if (!ok) break;
+        ode_step_body<true>(&sc, 1, p.slots, p.ranges, p.nranges, p.ntrack, p.cnt, p.stops, p.stop_node, p.cheb,
+                            s_cf);
+        __syncthreads();
+        if (sc.commit) {
+            yv = ynew;
+            sK[0][tid] = sK[12][tid];
+            const int smp = sc.sample;
+            if (smp >= 0) {
+                if (e < p.half) reinterpret_cast<cplx*>(p.Sf)[(long long)smp * p.half + e] = ynew;
+                else reinterpret_cast<cplx*>(p.Sa)[(long long)(p.c0.npts - 1 - smp) * p.half + (e - p.half)] = ynew;
+            }
+        }
+    }
+done:
+    reinterpret_cast<cplx*>(p.ybuf)[e] = yv;
+    if (wg == 0 && tid == 0) {
+        p.out[0] = ok ? (double)sc.status : (double)RC_ST_WAIT_LIMIT;
+        p.out[1] = (double)sc.attempts;
+        p.out[2] = (double)sc.accepted;
+        p.out[3] = (double)sc.rejected;
+        p.out[4] = (double)t_rhs;
+        p.out[5] = (double)t_sync;
+        p.out[6] = (double)t_ctl;
+        p.out[7] = (double)(__builtin_amdgcn_s_memrealtime() - t_start);
+    }
+}
+
+// workgroups of rc_quartic_ode that can be resident at once on the current device
+hipError_t rc_quartic_ode_capacity(int* blocks) {
+    int per_cu = 0, dev = 0, cus = 0, coop = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev)) != hipSuccess) return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)rc_quartic_ode, 256, 0)) != hipSuccess)
+        return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    *blocks = coop ? per_cu * cus : 0;
+    return hipSuccess;
+}
+
+hipError_t rc_launch_quartic_ode(dim3 grid, const RcOdeRun* dparams, hipStream_t stream) {
+    void* args[] = {(void*)&dparams};
+    return hipLaunchCooperativeKernel((const void*)rc_quartic_ode, grid, dim3(256), args, 0, stream);
 }
 
 // ---------------------------------------------------------------------------------------------

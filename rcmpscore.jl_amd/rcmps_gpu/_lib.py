@@ -137,6 +137,8 @@ SIGNATURES = {
 EXC_KINETIC = 1
 EXC_POTENTIAL = 2
 EXC_DENSE = 4
+EXC_GRAPH = 8
+EXC_PERSISTENT = 16
 
 _lock = threading.Lock()
 _lib = None

@@ -197,11 +197,14 @@ PRECONDITIONER_KRYLOV = KineticKrylovOptions(tolerance=1e-9, fallback_tolerance=
 
 
 def action_batch(ctx, ws, model: ExcitationModel, kinetic_nodes: int, parts: int = 0, return_stats: bool = False,
-                 krylov_options: KineticKrylovOptions | None = None, z2_blocks: bool = True):
+                 krylov_options: KineticKrylovOptions | None = None, z2_blocks: bool = True,
+                 persistent: bool | None = None):
     """F(W) for a list of D x D tangents in one device call (every kinetic Krylov iteration and ODE
     step shared); kinetic part on the fixed grid of `kinetic_nodes` theta nodes.  `ctx` is one
     ExcitationContext or one per tangent (momenta may differ; state and integration settings must
-    not).  z2_blocks=False makes the action ODE ignore a Z2 state's zero parity blocks (same bits)."""
+    not).  z2_blocks=False makes the action ODE ignore a Z2 state's zero parity blocks (same bits);
+    persistent=True integrates it in one persistent launch where every tile's workgroup fits,
+    False by replayed launch graphs, None takes the faster for the size (same bits)."""
     ws = [np.asarray(w, dtype=np.complex128) for w in ws]
     ctxs = list(ctx) if isinstance(ctx, (list, tuple)) else [ctx] * len(ws)
     if len(ctxs) != len(ws):
@@ -234,7 +237,9 @@ def action_batch(ctx, ws, model: ExcitationModel, kinetic_nodes: int, parts: int
     torch.cuda.synchronize(dev)
     st = _lib.load().rcmps_excitation_action_multi(_lib.context(s.device), n, hp, wp, fp, float(model.coupling), kc,
                                                    ctypes.byref(ip), int(kinetic_nodes),
-                                                   int(parts) | (0 if z2_blocks else _lib.EXC_DENSE),
+                                                   int(parts) | (0 if z2_blocks else _lib.EXC_DENSE)
+                                                   | {None: 0, True: _lib.EXC_PERSISTENT,
+                                                      False: _lib.EXC_GRAPH}[persistent],
                                                    ctypes.byref(stats))
     if st == 10:
         phase = {20: "ket Poisson solve", 21: "kinetic forward Krylov", 22: "kinetic adjoint Krylov",
