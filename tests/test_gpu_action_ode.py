@@ -79,3 +79,31 @@ def test_persistent_launch_gives_the_graph_bits(rg, case, ntan):
             assert np.array_equal(a, b)
     ctx.close()
     s.close()
+
+
+def test_step_limit_stops_both_routes_alike(rg):
+    """A step budget far below what the integration needs ends both routes with MAX_STEPS (the
+    controller's own stop, taken by every workgroup of the persistent launch at the same attempt),
+    and the context gives the same bits afterwards."""
+    note("case: action ODE step limit, D=16 checkpoint, both routes")
+    from rcmps_gpu import IntegratorParameters
+    from rcmps_gpu._lib import DeviceControlFailure
+    from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
+    s, g = state(rg, "phi4_g2_D16_z2_state")
+    ctx = ExcitationContext(s, 0.4)
+    model = ExcitationModel("phi4", g)
+    w = np.random.default_rng(9).standard_normal((16, 16)) + 0j
+    (before,) = action_batch(ctx, [w], model, 32, parts=2, persistent=True)
+    ip = ctx.integrator
+    ctx.integrator = IntegratorParameters(n_quad=ip.n_quad, max_point=ip.max_point, atol=ip.atol, rtol=ip.rtol,
+                                          maxiters=20)
+    for persistent in (True, False):
+        with pytest.raises(DeviceControlFailure) as exc:
+            action_batch(ctx, [w], model, 32, parts=2, persistent=persistent)
+        note(f"  persistent={persistent}: {exc.value}")
+        assert exc.value.status == 1
+    ctx.integrator = ip
+    (after,) = action_batch(ctx, [w], model, 32, parts=2, persistent=True)
+    assert np.array_equal(before, after)
+    ctx.close()
+    s.close()
