@@ -190,6 +190,7 @@ struct rcmps_context {
     size_t kry_pinned_doubles = 0;
     hipEvent_t kry_ev[2] = {nullptr, nullptr};
     int last_D = 0, last_Dp = 0, last_npts = 0;
+    int last_tb = 0;             // trajectory buffers of the last evaluation's first state
     bool last_adjoint = false;   // the last evaluation integrated the adjoint systems
     // quadrature cache: nodes (plus endpoint) and the pullback weights on the device
     int q_n = 0;
@@ -1238,17 +1239,19 @@ void build_state(rcmps_context* ctx, TrajBufs& tb, const rcmps_state* st, bool a
 }
 
 // Integrate every system of the given evaluations (all of one padded size Dp) in one persistent
-// launch; states are packed into rounds of at most the resident workgroup count.
+// launch; states are packed into rounds of at most the resident workgroup count.  Evaluation j
+// keeps its trajectories in ctx->tb[tb0 + j].
 void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, bool timers, bool z2,
-               const rcmps_integrator* ip, const std::vector<int>& idx, std::vector<EvalReq>& reqs) {
+               const rcmps_integrator* ip, const std::vector<int>& idx, size_t tb0, std::vector<EvalReq>& reqs) {
     const int NB = Dp / 16, npts = 2 * ip->n_quad + 1;
     std::vector<TrajSys> sys;
     std::vector<TrajTask> tasks;
-    while (ctx->tb.size() < idx.size()) ctx->tb.emplace_back(new TrajBufs());
+    while (ctx->tb.size() < tb0 + idx.size()) ctx->tb.emplace_back(new TrajBufs());
     std::vector<int> state_first, state_ntasks;
     for (size_t j = 0; j < idx.size(); ++j) {
         const int first = (int)tasks.size();
-        build_state(ctx, *ctx->tb[j], reqs[idx[j]].st, reqs[idx[j]].adjoint(), quartic_only, z2, npts, sys, tasks);
+        build_state(ctx, *ctx->tb[tb0 + j], reqs[idx[j]].st, reqs[idx[j]].adjoint(), quartic_only, z2, npts, sys,
+                    tasks);
         state_first.push_back(first);
         state_ntasks.push_back((int)tasks.size() - first);
     }
@@ -1297,8 +1300,8 @@ void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, bool t
     ck(hipMemcpyAsync(ctx->traj_sched.p, sched.data(), sched.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     ck(hipMemcpyAsync(ctx->traj_params.p, &P, sizeof(TrajParams), hipMemcpyHostToDevice, ctx->stream));
     for (size_t j = 0; j < idx.size(); ++j) {
-        ck(hipMemsetAsync(ctx->tb[j]->flags.p, 0, ctx->tb[j]->flag_bytes, ctx->stream));
-        ck(hipMemsetAsync(ctx->tb[j]->out.p, 0, RC_NSYS * RC_OUT_STRIDE * 8, ctx->stream));
+        ck(hipMemsetAsync(ctx->tb[tb0 + j]->flags.p, 0, ctx->tb[tb0 + j]->flag_bytes, ctx->stream));
+        ck(hipMemsetAsync(ctx->tb[tb0 + j]->out.p, 0, RC_NSYS * RC_OUT_STRIDE * 8, ctx->stream));
     }
     ck(hipMemsetAsync(ctx->abort_word.p, 0, 256, ctx->stream));
     ck(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -1307,8 +1310,8 @@ void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, bool t
     ck(hipEventRecord(ctx->ev[1], ctx->stream));
     std::vector<double> outs(idx.size() * RC_NSYS * RC_OUT_STRIDE);
     for (size_t j = 0; j < idx.size(); ++j)
-        ck(hipMemcpyAsync(outs.data() + j * RC_NSYS * RC_OUT_STRIDE, ctx->tb[j]->out.d(), RC_NSYS * RC_OUT_STRIDE * 8,
-                          hipMemcpyDeviceToHost, ctx->stream));
+        ck(hipMemcpyAsync(outs.data() + j * RC_NSYS * RC_OUT_STRIDE, ctx->tb[tb0 + j]->out.d(),
+                          RC_NSYS * RC_OUT_STRIDE * 8, hipMemcpyDeviceToHost, ctx->stream));
     ck(hipStreamSynchronize(ctx->stream));
     float tms = 0;
     ck(hipEventElapsedTime(&tms, ctx->ev[0], ctx->ev[1]));
@@ -1341,6 +1344,7 @@ void run_class(rcmps_context* ctx, int Dp, bool split, bool quartic_only, bool t
     if (!idx.empty()) {
         const rcmps_state* st0 = reqs[idx[0]].st;
         ctx->last_D = st0->D; ctx->last_Dp = st0->Dp; ctx->last_npts = npts;
+        ctx->last_tb = (int)tb0;
         ctx->last_adjoint = reqs[idx[0]].adjoint();
     }
 }
@@ -2035,21 +2039,27 @@ bool valid_integrator(const rcmps_integrator* ip) {
 
 // A batch of evaluations: grouped by padded size, one persistent launch per group, then the
 // pullback of each evaluation.  Every request gets its stats; the first failure is returned.
-// The per-state pullback, boundary and metric solves of one class on up to kFinishWorkers worker
-// contexts, each driven by its own host thread and stream (the states' trajectories are complete
-// and the main stream idle).  Each state is finished by one worker with the same kernels and tile
-// choices as on the main context (every tile variant gives the same bits), so the results equal
-// the sequential ones bit for bit.  Returns false (nothing done) for a single state or when a state
-// appears twice (its cached metric inverse would be shared between threads).
+// The per-state pullback, boundary and metric solves of one class run on up to kFinishWorkers worker
+// contexts, each driven by its own host thread and stream, while the main stream integrates the
+// next class.  Each state is finished by one worker with the same kernels and tile choices as on the
+// main context (every tile variant gives the same bits), so the results equal the sequential ones
+// bit for bit.  Not for a single state or when a state appears twice (its cached metric inverse
+// would be shared between threads).
 constexpr int kFinishWorkers = 4;
 
-bool finish_concurrently(rcmps_context* ctx, const rcmps_integrator* ip, double metric_reg, int flags,
-                         const std::vector<int>& idx, std::vector<EvalReq>& reqs) {
+bool can_finish_concurrently(const std::vector<int>& idx, const std::vector<EvalReq>& reqs) {
     if (idx.size() < 2) return false;
     std::vector<const rcmps_state*> seen;
     for (int i : idx) seen.push_back(reqs[i].st);
     std::sort(seen.begin(), seen.end());
-    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) return false;
+    return std::adjacent_find(seen.begin(), seen.end()) == seen.end();
+}
+
+// Finish the evaluations idx (trajectories in ctx->tb[tb0 + j]) on the worker contexts.  Only this
+// function touches ctx->workers; it may run on a host thread of its own while the main thread
+// drives ctx->stream.
+void finish_concurrently(rcmps_context* ctx, const rcmps_integrator* ip, double metric_reg, int flags,
+                         const std::vector<int>& idx, size_t tb0, std::vector<EvalReq>& reqs) {
     const int nw = std::min<int>(kFinishWorkers, (int)idx.size());
     while ((int)ctx->workers.size() < nw) {
         rcmps_context* w = nullptr;
@@ -2074,7 +2084,7 @@ bool finish_concurrently(rcmps_context* ctx, const rcmps_integrator* ip, double 
             int code = RCMPS_OK;
             try {
                 code = guard([&] {
-                    finish_state(wc, r, *ctx->tb[j], ip, metric_reg, flags);
+                    finish_state(wc, r, *ctx->tb[tb0 + j], ip, metric_reg, flags);
                     ck(hipStreamSynchronize(wc->stream));
                 });
             } catch (...) {   // nothing may leave a worker thread
@@ -2087,9 +2097,11 @@ bool finish_concurrently(rcmps_context* ctx, const rcmps_integrator* ip, double 
     for (int w = 1; w < nw; ++w) threads.emplace_back(work, ctx->workers[w]);
     work(ctx->workers[0]);
     for (auto& t : threads) t.join();
-    return true;
 }
 
+// The classes run largest first, so that the longest finishing overlaps the most integration.  A
+// class finished on the workers keeps its trajectory buffers until it is done, so consecutive
+// classes alternate between two sets of buffers.
 int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcmps_integrator* ip,
                        double metric_reg, int flags) {
     if (!ctx || !valid_integrator(ip) || !(metric_reg >= 0)) return RCMPS_ERR_ARGUMENT;
@@ -2102,17 +2114,50 @@ int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcm
         std::memset(r.S, 0, sizeof(*r.S));
     }
     auto t_start = std::chrono::steady_clock::now();
+    std::thread finisher;
+    std::exception_ptr finish_error;
+    auto join_finisher = [&] {
+        if (finisher.joinable()) finisher.join();
+        if (finish_error) {
+            std::exception_ptr e = finish_error;
+            finish_error = nullptr;
+            std::rethrow_exception(e);
+        }
+    };
     int rc = guard([&] {
+        try {
         ck(hipSetDevice(ctx->device));
         ensure_quadrature(ctx, ip);
-        for (int Dp = 16; Dp <= 64; Dp += 16) {
+        std::vector<std::vector<int>> classes;
+        size_t largest = 0;
+        for (int Dp = 64; Dp >= 16; Dp -= 16) {
             std::vector<int> idx;
             for (size_t i = 0; i < reqs.size(); ++i)
                 if (reqs[i].st->Dp == Dp) idx.push_back((int)i);
             if (idx.empty()) continue;
+            largest = std::max(largest, idx.size());
+            classes.push_back(std::move(idx));
+        }
+        const bool overlap = !krylov && classes.size() > 1;
+        // sized before any class starts: a worker thread reads ctx->tb while the next class runs
+        while (ctx->tb.size() < (overlap ? 2 : 1) * largest) ctx->tb.emplace_back(new TrajBufs());
+        for (size_t c = 0; c < classes.size(); ++c) {
+            const std::vector<int>& idx = classes[c];
+            const int Dp = reqs[idx[0]].st->Dp;
+            const size_t tb0 = overlap ? (c & 1) * largest : 0;
             run_class(ctx, Dp, (flags & RCMPS_FLAG_SPLIT) != 0, krylov, (flags & RCMPS_FLAG_PHASE_TIMERS) != 0,
-                      (flags & RCMPS_FLAG_DENSE) == 0, ip, idx, reqs);
-            if (!krylov && finish_concurrently(ctx, ip, metric_reg, flags, idx, reqs)) continue;
+                      (flags & RCMPS_FLAG_DENSE) == 0, ip, idx, tb0, reqs);
+            join_finisher();   // the previous class's finishing, overlapped with this class's launch
+            if (!krylov && can_finish_concurrently(idx, reqs)) {
+                finisher = std::thread([&, c, tb0] {
+                    try {
+                        finish_concurrently(ctx, ip, metric_reg, flags, classes[c], tb0, reqs);
+                    } catch (...) {
+                        finish_error = std::current_exception();
+                    }
+                });
+                continue;
+            }
             for (size_t j = 0; j < idx.size(); ++j) {
                 EvalReq& r = reqs[idx[j]];
                 if (r.S->status != RCMPS_OK) continue;
@@ -2132,10 +2177,15 @@ int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcm
                         kc = kcov;
                         ke = kenv;
                     }
-                    if (r.adjoint()) finish_state(ctx, r, *ctx->tb[j], ip, metric_reg, flags, kc, ke);
+                    if (r.adjoint()) finish_state(ctx, r, *ctx->tb[tb0 + j], ip, metric_reg, flags, kc, ke);
                 });
                 if (code != RCMPS_OK) r.S->status = code;
             }
+        }
+        join_finisher();
+        } catch (...) {
+            if (finisher.joinable()) finisher.join();   // never leave a running thread behind
+            throw;
         }
     });
     const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -2289,7 +2339,7 @@ int rcmps_trajectory_samples(rcmps_context* ctx, int system, double* dst, long l
     const long long need = (long long)ctx->last_npts * nl * D * D * 2;
     if (capacity < need) return RCMPS_ERR_ARGUMENT;
     return guard([&] {
-        const double* src = ctx->tb[0]->samples[system].d();
+        const double* src = ctx->tb[ctx->last_tb]->samples[system].d();
         const int rows = ctx->last_npts * nl;
         for (int r = 0; r < rows; ++r)
             pad_out(ctx, D, Dp, src + (size_t)r * Dp * Dp * 2, dst + (size_t)r * D * D * 2);
