@@ -114,7 +114,7 @@ def batch_main(args):
     sp, fp = result["split"], result["fp64"]
     n = args.batch
     out = {
-        "metric": "batched fg_phi4 E+grad evals/s (uneven D and g, one persistent launch per padded size)",
+        "metric": "batched fg_phi4 E+grad evals/s (uneven D and g, persistent launches by padded size)",
         "value": n * args.steps / sp["elapsed"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * sp["elapsed"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 (split: int8 MFMA over exact digit slices)",

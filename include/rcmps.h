@@ -149,9 +149,11 @@ int rcmps_fg_phi4(rcmps_context* ctx, const rcmps_state* st, double g, const rcm
 
 /* fg_phi4 over n independent states (any mix of bond dimensions and couplings g[i]) with one
    integrator: energies[i] (host) and, where grads and grads[i] are non-null, the Riemannian
-   gradient into the device buffer grads[i] (D_i x D_i).  States of the same padded size share one
+   gradient into the device buffer grads[i] (D_i x D_i).  States of the same padded size share a
    persistent launch in which their trajectories are gang-scheduled in rounds of resident
-   workgroups.  stats may be null or an array of n.  Returns the first failure (each stats[i]
+   workgroups (the largest size in two launches); each launch's pullback, boundary and metric
+   solves run on worker streams while the next launch integrates.  stats may be null or an array
+   of n.  Returns the first failure (each stats[i]
    holds its own status); the evaluations that succeeded are complete.  Replaces a loop of
    fg_phi4(s_i, g_i) calls (src/observables.jl:257); the reference has no batched form. */
 int rcmps_fg_phi4_batch(rcmps_context* ctx, int n, const rcmps_state* const* states, const double* g,

@@ -2099,9 +2099,11 @@ void finish_concurrently(rcmps_context* ctx, const rcmps_integrator* ip, double 
     for (auto& t : threads) t.join();
 }
 
-// The classes run largest first, so that the longest finishing overlaps the most integration.  A
-// class finished on the workers keeps its trajectory buffers until it is done, so consecutive
-// classes alternate between two sets of buffers.
+// The classes run smallest first: a class's finishing then runs under a longer launch, and only the
+// last class's is left.  That one is cut in two launches, the second of about an eighth of its
+// states, so that the first part finishes under the second launch.  Results do not depend on
+// which launch integrates a state.  A class finished on the workers keeps its trajectory buffers
+// until it is done, so consecutive launches alternate between two sets of buffers.
 int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcmps_integrator* ip,
                        double metric_reg, int flags) {
     if (!ctx || !valid_integrator(ip) || !(metric_reg >= 0)) return RCMPS_ERR_ARGUMENT;
@@ -2129,15 +2131,21 @@ int fg_phi4_batch_impl(rcmps_context* ctx, std::vector<EvalReq>& reqs, const rcm
         ck(hipSetDevice(ctx->device));
         ensure_quadrature(ctx, ip);
         std::vector<std::vector<int>> classes;
-        size_t largest = 0;
-        for (int Dp = 64; Dp >= 16; Dp -= 16) {
+        for (int Dp = 16; Dp <= 64; Dp += 16) {
             std::vector<int> idx;
             for (size_t i = 0; i < reqs.size(); ++i)
                 if (reqs[i].st->Dp == Dp) idx.push_back((int)i);
-            if (idx.empty()) continue;
-            largest = std::max(largest, idx.size());
-            classes.push_back(std::move(idx));
+            if (!idx.empty()) classes.push_back(std::move(idx));
         }
+        if (!krylov && !classes.empty() && classes.back().size() >= 8) {
+            std::vector<int>& last = classes.back();
+            const size_t tail = (last.size() + 7) / 8;
+            std::vector<int> second(last.end() - tail, last.end());
+            last.resize(last.size() - tail);
+            classes.push_back(std::move(second));
+        }
+        size_t largest = 0;
+        for (const auto& idx : classes) largest = std::max(largest, idx.size());
         const bool overlap = !krylov && classes.size() > 1;
         // sized before any class starts: a worker thread reads ctx->tb while the next class runs
         while (ctx->tb.size() < (overlap ? 2 : 1) * largest) ctx->tb.emplace_back(new TrajBufs());
