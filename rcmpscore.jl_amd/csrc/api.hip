@@ -75,13 +75,13 @@ struct KrylovBasis {
 // workspace of the infinite-line kinetic backend
 struct KineticWork {
     KrylovBasis basis[4];     // (op R, op [Q,R]) x (forward, adjoint)
-    DevBuf X, A, LX, T, Yd, zd, wd, wzd, res, hd, tmp;
+    DevBuf X, A, X2, A2, rhs2, w4, hd4, LX, T, Yd, zd, wd, wzd, res, hd, tmp;
     int actions = 0;
     double max_res = 0.0;
     double ms[4] = {0.0, 0.0, 0.0, 0.0};   // Arnoldi, projected solves, solutions + residuals, total
     void release() {
         for (auto& b : basis) { b.V.release(); b = KrylovBasis(); }
-        DevBuf* all[] = {&X, &A, &LX, &T, &Yd, &zd, &wd, &wzd, &res, &hd, &tmp};
+        DevBuf* all[] = {&X, &A, &X2, &A2, &rhs2, &w4, &hd4, &LX, &T, &Yd, &zd, &wd, &wzd, &res, &hd, &tmp};
         for (auto* d : all) d->release();
     }
 };
@@ -1622,56 +1622,89 @@ void basis_start(rcmps_context* ctx, Dev& dv, KrylovBasis& b, const double* rhs,
     dv.axpby(cd(1.0 / b.beta, 0), rhs, cd(0, 0), nullptr, b.V.d());
 }
 
-// Arnoldi steps up to `target` vectors, classical Gram-Schmidt twice, H from both passes
-// (kinetic_krylov.jl:68-92); exhaustion at a vanishing new direction or the full dimension D^2.
-// No step waits for the host: each writes its Hessenberg column (|w|^2 before, the two passes'
-// projections, |w|^2 after) to its own slot and normalises on the device, and the host reads all
-// columns once.  A step past an exhaustion divides by a vanishing norm; its vector and column are
-// discarded (the basis keeps the vectors up to the exhaustion), so the kept bits are those of a
-// step-by-step loop.
-void basis_extend(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasis& b, bool adj, int target) {
+// Arnoldi steps of up to four bases at once, basis i up to targets[i] vectors: classical Gram-Schmidt
+// twice, H from both passes (kinetic_krylov.jl:68-92); exhaustion at a vanishing new direction or the
+// full dimension D^2.  Every launch serves all bases still extending (rc_lindblad_multi, rc_arn_*), and
+// each basis gets the vectors, Hessenberg columns and exhaustion it gets when extended alone.  No step
+// waits for the host: each writes its Hessenberg column (|w|^2 before, the two passes' projections,
+// |w|^2 after) to its own slot and normalises on the device, and the host reads all columns once.  A
+// step past an exhaustion divides by a vanishing norm; its vector and column are discarded (the basis
+// keeps the vectors up to the exhaustion), so the kept bits are those of a step-by-step loop.
+void basis_extend_multi(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasis* const* bs, const bool* adj,
+                        const int* targets, int nb) {
     KineticWork& kw = ctx->kin;
     const int n = dv.n();
     const int D2 = st->D * st->D;
-    target = std::min(target, D2);
-    basis_grow(b, n, target, ctx->stream);
-    if (target <= b.k) return;
-    const int nsteps = target - b.k;
-    const size_t stride = 2 * (size_t)(b.cap + 1) + 2;   // complex entries per step's column slot
-    ck(kw.hd.ensure((size_t)nsteps * stride * 16));
-    double* V = b.V.d();
-    double* w = ctx->kry_w.d();
-    for (int j = b.k; j < target; ++j) {
-        double* hd0 = kw.hd.d() + 2 * (size_t)(j - b.k) * stride;
-        double* hd1 = hd0 + 2;
-        double* hd2 = hd1 + 2 * (b.cap + 1);
-        double* hnn = hd2 + 2 * (b.cap + 1);
-        lindblad(dv, st, V + (size_t)j * n * 2, w, adj, ctx->kry_t1.d());
-        hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hd0);
-        hipLaunchKernelGGL(rc_dots, dim3(j + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd1);
-        hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, j + 1, hd1, n, w);
-        hipLaunchKernelGGL(rc_dots, dim3(j + 1), dim3(256), 0, ctx->stream, V, (long long)n, n, w, hd2);
-        hipLaunchKernelGGL(rc_subtract_combo, dv.eb(), dim3(256), 0, ctx->stream, V, (long long)n, j + 1, hd2, n, w);
-        hipLaunchKernelGGL(rc_dots, dim3(1), dim3(256), 0, ctx->stream, w, 0LL, n, w, hnn);
-        hipLaunchKernelGGL(rc_normalize, dv.eb(), dim3(256), 0, ctx->stream, n, w, hnn, V + (size_t)(j + 1) * n * 2);
+    int tg[4], k0[4], nsteps[4], maxsteps = 0;
+    size_t stride[4], off[4], total = 0;
+    for (int i = 0; i < nb; ++i) {
+        tg[i] = std::min(targets[i], D2);
+        basis_grow(*bs[i], n, tg[i], ctx->stream);
+        k0[i] = bs[i]->k;
+        nsteps[i] = std::max(0, tg[i] - k0[i]);
+        stride[i] = 2 * (size_t)(bs[i]->cap + 1) + 2;   // complex entries per step's column slot
+        off[i] = total;
+        total += (size_t)nsteps[i] * stride[i];
+        maxsteps = std::max(maxsteps, nsteps[i]);
+    }
+    if (maxsteps == 0) return;
+    ck(kw.hd4.ensure(total * 16));
+    ck(kw.w4.ensure((size_t)nb * n * 16));
+    for (int s = 0; s < maxsteps; ++s) {
+        RcArnBatch B{};
+        RcLindbladMulti L{};
+        B.n = n;
+        int maxj = 0;
+        for (int i = 0; i < nb; ++i) {
+            if (s >= nsteps[i]) continue;
+            const int j = k0[i] + s;
+            RcArnBasis& a = B.b[B.nb];
+            a.V = bs[i]->V.d();
+            a.w = kw.w4.d() + 2 * (size_t)i * n;
+            a.wl = a.w;
+            a.h0 = kw.hd4.d() + 2 * (off[i] + (size_t)s * stride[i]);
+            a.h1 = a.h0 + 2;
+            a.h2 = a.h1 + 2 * (bs[i]->cap + 1);
+            a.hn = a.h2 + 2 * (bs[i]->cap + 1);
+            a.j = j;
+            L.X[B.nb] = a.V + (size_t)j * n * 2;
+            L.out[B.nb] = a.w;
+            L.adj[B.nb] = adj[i] ? 1 : 0;
+            maxj = std::max(maxj, j);
+            ++B.nb;
+        }
+        if (dv.Dp <= 64) {
+            const int nbt = dv.Dp / 16;
+            hipLaunchKernelGGL(rc_lindblad_multi, dim3(nbt * nbt, B.nb), dim3(256), 0, ctx->stream, dv.Dp, st->Q.d(),
+                               st->Qh.d(), st->R.d(), st->Rh.d(), L);
+        } else {
+            for (int q = 0; q < B.nb; ++q) lindblad(dv, st, L.X[q], L.out[q], L.adj[q] != 0, ctx->kry_t1.d());
+        }
+        hipLaunchKernelGGL(rc_arn_dots, dim3(maxj + 2, B.nb), dim3(256), 0, ctx->stream, B, 1);
+        hipLaunchKernelGGL(rc_arn_subtract, dim3(ceil_div(n, 256), B.nb), dim3(256), 0, ctx->stream, B, 1);
+        hipLaunchKernelGGL(rc_arn_dots, dim3(maxj + 1, B.nb), dim3(256), 0, ctx->stream, B, 2);
+        hipLaunchKernelGGL(rc_arn_subtract, dim3(ceil_div(n, 256), B.nb), dim3(256), 0, ctx->stream, B, 2);
+        hipLaunchKernelGGL(rc_arn_normalize, dim3(1, B.nb), dim3(256), 0, ctx->stream, B);
         ck(hipGetLastError());
     }
-    std::vector<cd> hv((size_t)nsteps * stride);
-    ck(hipMemcpyAsync(hv.data(), kw.hd.d(), hv.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<cd> hv(total);
+    ck(hipMemcpyAsync(hv.data(), kw.hd4.d(), total * 16, hipMemcpyDeviceToHost, ctx->stream));
     ck(hipStreamSynchronize(ctx->stream));
-    const int k0 = b.k;
-    for (int j = k0; j < target; ++j) {
-        const cd* col = hv.data() + (size_t)(j - k0) * stride;
-        ++kw.actions;
-        const double oldnorm = norm_sqrt(col[0].real());
-        for (int i = 0; i <= j; ++i) b.H[(size_t)i * b.cap + j] += col[1 + i] + col[1 + (b.cap + 1) + i];
-        const double beta = norm_sqrt(col[1 + 2 * (b.cap + 1)].real());
-        if (!std::isfinite(oldnorm) || !std::isfinite(beta)) throw Fail{RCMPS_ERR_SOLVER};
-        b.H[(size_t)(j + 1) * b.cap + j] = beta;
-        b.k = j + 1;
-        if (beta <= 10 * 2.220446049250313e-16 * std::max(oldnorm, 2.220446049250313e-16) || j + 1 == D2) {
-            b.exhausted = true;
-            break;
+    for (int i = 0; i < nb; ++i) {
+        KrylovBasis& b = *bs[i];
+        for (int j = k0[i]; j < tg[i]; ++j) {
+            const cd* col = hv.data() + off[i] + (size_t)(j - k0[i]) * stride[i];
+            ++kw.actions;
+            const double oldnorm = norm_sqrt(col[0].real());
+            for (int r = 0; r <= j; ++r) b.H[(size_t)r * b.cap + j] += col[1 + r] + col[1 + (b.cap + 1) + r];
+            const double beta = norm_sqrt(col[1 + 2 * (b.cap + 1)].real());
+            if (!std::isfinite(oldnorm) || !std::isfinite(beta)) throw Fail{RCMPS_ERR_SOLVER};
+            b.H[(size_t)(j + 1) * b.cap + j] = beta;
+            b.k = j + 1;
+            if (beta <= 10 * 2.220446049250313e-16 * std::max(oldnorm, 2.220446049250313e-16) || j + 1 == D2) {
+                b.exhausted = true;
+                break;
+            }
         }
     }
 }
@@ -1749,90 +1782,135 @@ bool projected_solutions(const KrylovBasis& b, const std::vector<double>& z, std
 }
 
 // Shifted solutions X_i = (I - z_i L)^{-1} rhs (or L*) for all nodes, each accepted on its true
-// relative residual (kinetic_krylov.jl:114-155).  X: nz matrices at stride Dp^2.
+// relative residual (kinetic_krylov.jl:114-155).  X: nz matrices at stride Dp^2.  Every job (one
+// basis, its right-hand side and direction) runs its own sequence of tests; the extensions the
+// jobs ask for in one round run together (basis_extend_multi), so each job's bits are those of
+// running it alone.
 constexpr int kProjectedThreads = 16;
 
-void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KrylovBasis& b, const double* rhs, bool adj,
-                       const std::vector<double>& z, double* X) {
+struct KinJob {
+    KrylovBasis* b;
+    const double* rhs;
+    bool adj;
+    double* X;
+    int want;      // the basis size this job waits for
+    bool done;
+};
+
+// One pass of a job's test sequence: it returns done, or with the basis size it needs first.
+void kinetic_advance(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KinJob& jb, const std::vector<double>& z,
+                     int zhard) {
     KineticWork& kw = ctx->kin;
     const rcmps_kinetic_options& o = ctx->kopt;
+    KrylovBasis& b = *jb.b;
     const int n = dv.n(), nz = (int)z.size();
     const int maxdim = std::min(o.maxdim, st->D * st->D), initdim = std::min(o.initialdim, maxdim);
-    if (!b.started) basis_start(ctx, dv, b, rhs, initdim);
-    if (b.beta == 0.0) {
-        ck(hipMemsetAsync(X, 0, (size_t)nz * n * 16, ctx->stream));
-        return;
-    }
-    ck(kw.zd.ensure((size_t)nz * 8));
-    ck(kw.res.ensure((size_t)nz * 8));
-    ck(hipMemcpyAsync(kw.zd.d(), z.data(), (size_t)nz * 8, hipMemcpyHostToDevice, ctx->stream));
+    const std::vector<double> zone(1, z[zhard]);
     std::vector<cd> Y;
     std::vector<double> res(nz);
-    // the node of largest z alone first: when its projected residual (one term of the full test)
-    // misses the tolerance on a basis that can grow, the basis extends without the other nodes' solves
-    int zhard = 0;
-    for (int i = 1; i < nz; ++i)
-        if (std::fabs(z[i]) > std::fabs(z[zhard])) zhard = i;
-    const std::vector<double> zone(1, z[zhard]);
     using clock = std::chrono::steady_clock;
     auto since = [](clock::time_point t0) { return std::chrono::duration<double, std::milli>(clock::now() - t0).count(); };
-    auto extend = [&](int target) {
-        const auto t0 = clock::now();
-        basis_extend(ctx, st, dv, b, adj, target);
-        kw.ms[0] += since(t0);
-    };
     auto project = [&](const std::vector<double>& zz, int threads, double& e) {
         const auto t0 = clock::now();
         const bool r = projected_solutions(b, zz, Y, e, threads);
         kw.ms[1] += since(t0);
         return r;
     };
-    while (true) {
-        if (b.k == 0) extend(initdim);
-        const int k = b.k;
-        const bool terminal = b.exhausted || k >= maxdim;
-        double est = 0;
-        if (!terminal && nz > 1) {
-            const bool ok1 = project(zone, 1, est);
-            if (!ok1 || est > o.tolerance) {
-                extend(std::min(2 * k, maxdim));
-                continue;
-            }
+    jb.want = 0;
+    if (b.k == 0) {
+        jb.want = initdim;
+        return;
+    }
+    const int k = b.k;
+    const bool terminal = b.exhausted || k >= maxdim;
+    double est = 0;
+    // the node of largest z alone first: when its projected residual (one term of the full test)
+    // misses the tolerance on a basis that can grow, the basis extends without the other nodes' solves
+    if (!terminal && nz > 1) {
+        const bool ok1 = project(zone, 1, est);
+        if (!ok1 || est > o.tolerance) {
+            jb.want = std::min(2 * k, maxdim);
+            return;
         }
-        const bool ok = project(z, kProjectedThreads, est);
-        if (ok && (est <= o.tolerance || terminal)) {
-            const auto t0 = clock::now();
-            ck(kw.Yd.ensure(Y.size() * 16));
-            ck(hipMemcpyAsync(kw.Yd.p, Y.data(), Y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(rc_basis_combine, dim3(ceil_div(n, 256), ceil_div(nz, 8)), dim3(256), 0, ctx->stream,
-                               b.V.d(), (long long)n, k, kw.Yd.d(), nz, n, X);
-            ck(hipGetLastError());
-            lindblad_batched(dv, st, X, kw.LX.d(), kw.T.d(), nz, adj);
-            hipLaunchKernelGGL(rc_shifted_residual, dim3(nz), dim3(256), 0, ctx->stream, X, kw.LX.d(), rhs, kw.zd.d(), n,
-                               kw.res.d());
-            ck(hipGetLastError());
-            ck(hipMemcpyAsync(res.data(), kw.res.d(), (size_t)nz * 8, hipMemcpyDeviceToHost, ctx->stream));
-            ck(hipStreamSynchronize(ctx->stream));
-            kw.ms[2] += since(t0);
-            double worst = 0;
-            for (double r2 : res) worst = std::max(worst, std::sqrt(r2) / b.beta);
-            const bool fallback = terminal && worst > o.tolerance && worst <= o.fallback_tolerance;
-            if (std::isfinite(worst) && (worst <= o.tolerance || fallback)) {
-                kw.max_res = std::max(kw.max_res, worst);
-                return;
-            }
-            if (b.exhausted) throw Fail{RCMPS_ERR_SOLVER};
+    }
+    const bool ok = project(z, kProjectedThreads, est);
+    if (ok && (est <= o.tolerance || terminal)) {
+        const auto t0 = clock::now();
+        ck(kw.Yd.ensure(Y.size() * 16));
+        ck(hipMemcpyAsync(kw.Yd.p, Y.data(), Y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(rc_basis_combine, dim3(ceil_div(n, 256), ceil_div(nz, 8)), dim3(256), 0, ctx->stream,
+                           b.V.d(), (long long)n, k, kw.Yd.d(), nz, n, jb.X);
+        ck(hipGetLastError());
+        lindblad_batched(dv, st, jb.X, kw.LX.d(), kw.T.d(), nz, jb.adj);
+        hipLaunchKernelGGL(rc_shifted_residual, dim3(nz), dim3(256), 0, ctx->stream, jb.X, kw.LX.d(), jb.rhs,
+                           kw.zd.d(), n, kw.res.d());
+        ck(hipGetLastError());
+        ck(hipMemcpyAsync(res.data(), kw.res.d(), (size_t)nz * 8, hipMemcpyDeviceToHost, ctx->stream));
+        ck(hipStreamSynchronize(ctx->stream));
+        kw.ms[2] += since(t0);
+        double worst = 0;
+        for (double r2 : res) worst = std::max(worst, std::sqrt(r2) / b.beta);
+        const bool fallback = terminal && worst > o.tolerance && worst <= o.fallback_tolerance;
+        if (std::isfinite(worst) && (worst <= o.tolerance || fallback)) {
+            kw.max_res = std::max(kw.max_res, worst);
+            jb.done = true;
+            return;
         }
-        if (terminal) throw Fail{RCMPS_ERR_SOLVER};
-        extend(std::min(2 * k, maxdim));
+        if (b.exhausted) throw Fail{RCMPS_ERR_SOLVER};
+    }
+    if (terminal) throw Fail{RCMPS_ERR_SOLVER};
+    jb.want = std::min(2 * k, maxdim);
+}
+
+void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, std::vector<KinJob>& jobs,
+                       const std::vector<double>& z) {
+    KineticWork& kw = ctx->kin;
+    const rcmps_kinetic_options& o = ctx->kopt;
+    const int n = dv.n(), nz = (int)z.size();
+    const int maxdim = std::min(o.maxdim, st->D * st->D), initdim = std::min(o.initialdim, maxdim);
+    if (jobs.size() > 4) throw Fail{RCMPS_ERR_ARGUMENT};
+    for (KinJob& jb : jobs) {
+        jb.want = 0;
+        jb.done = false;
+        KrylovBasis& b = *jb.b;
+        if (!b.started) basis_start(ctx, dv, b, jb.rhs, initdim);
+        if (b.beta == 0.0) {
+            ck(hipMemsetAsync(jb.X, 0, (size_t)nz * n * 16, ctx->stream));
+            jb.done = true;
+        }
+    }
+    ck(kw.zd.ensure((size_t)nz * 8));
+    ck(kw.res.ensure((size_t)nz * 8));
+    ck(hipMemcpyAsync(kw.zd.d(), z.data(), (size_t)nz * 8, hipMemcpyHostToDevice, ctx->stream));
+    int zhard = 0;
+    for (int i = 1; i < nz; ++i)
+        if (std::fabs(z[i]) > std::fabs(z[zhard])) zhard = i;
+    for (;;) {
+        KrylovBasis* bs[4];
+        bool adj[4];
+        int tg[4], nw = 0;
+        for (KinJob& jb : jobs) {
+            if (jb.done) continue;
+            kinetic_advance(ctx, st, dv, jb, z, zhard);
+            if (jb.done) continue;
+            bs[nw] = jb.b;
+            adj[nw] = jb.adj;
+            tg[nw] = jb.want;
+            ++nw;
+        }
+        if (nw == 0) return;
+        const auto t0 = std::chrono::steady_clock::now();
+        basis_extend_multi(ctx, st, dv, bs, adj, tg, nw);
+        kw.ms[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 }
 
-// One kinetic spec on the infinite line: value, covector and environment (kinetic_krylov.jl:156-189).
+// One kinetic spec on the infinite line: value, covector and environment (kinetic_krylov.jl:156-189)
+// from its node solutions, X the forward solves on rho op^+ and A the adjoint solves on op^+.
 // Scratch: 12 Dp x Dp matrices at kw.tmp.
 void kinetic_components(rcmps_context* ctx, const rcmps_state* st, Dev& dv, int commutator,
                         const std::vector<double>& theta, const std::vector<double>& w, bool gradient, double& value,
-                        double* cov, double* env) {
+                        double* cov, double* env, const double* X, const double* A) {
     KineticWork& kw = ctx->kin;
     const int nz = (int)theta.size();
     const long long node = (long long)dv.n();
@@ -1844,7 +1922,6 @@ void kinetic_components(rcmps_context* ctx, const rcmps_state* st, Dev& dv, int 
     ck(hipMemcpyAsync(kw.wd.d(), w.data(), (size_t)nz * 8, hipMemcpyHostToDevice, ctx->stream));
     ck(hipMemcpyAsync(kw.wzd.d(), wz.data(), (size_t)nz * 8, hipMemcpyHostToDevice, ctx->stream));
     double* t = kw.tmp.d();
-    double* rhs = t;
     double* u = t + mcount;
     double* S1 = t + 2 * mcount;
     double* S2 = t + 3 * mcount;
@@ -1858,36 +1935,32 @@ void kinetic_components(rcmps_context* ctx, const rcmps_state* st, Dev& dv, int 
     double* x2 = t + 11 * mcount;
     const double* op = commutator ? st->C.d() : st->R.d();
     const double* opd = commutator ? st->Ch.d() : st->Rh.d();
-    // forward solves on rho op^+
-    dv.gemm(0, st->rho.d(), 0, opd, rhs);
-    kinetic_solutions(ctx, st, dv, kw.basis[2 * commutator], rhs, false, z, kw.X.d());
-    hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, kw.X.d(), node, kw.wd.d(), nz, dv.n(), 0.0, u);
+    hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, X, node, kw.wd.d(), nz, dv.n(), 0.0, u);
     ck(hipGetLastError());
     dv.trace_prod(op, u, ctx->small.d());
     value = dv.fetch2(ctx->small.d()).real();   // sum_i w_i Re tr(op x_i)
     if (!std::isfinite(value)) throw Fail{RCMPS_ERR_NONFINITE};
     if (!gradient) return;
-    // adjoint solves on op^+; a_i = A_i^+
-    kinetic_solutions(ctx, st, dv, kw.basis[2 * commutator + 1], opd, true, z, kw.A.d());
+    // a_i = A_i^+
     GemmArgs mb{};   // T_i = R^+ A_i^+
-    mb.A = st->Rh.d(); mb.opB = 1; mb.B = kw.A.d(); mb.strideB = node; mb.C = kw.T.d(); mb.strideC = node;
+    mb.A = st->Rh.d(); mb.opB = 1; mb.B = A; mb.strideB = node; mb.C = kw.T.d(); mb.strideC = node;
     dv.batched(mb, nz);
     GemmArgs s1{};   // S1 = sum wz x_i T_i
-    s1.A = kw.X.d(); s1.strideA = node; s1.B = kw.T.d(); s1.strideB = node; s1.alpha = kw.wzd.d(); s1.alpha_stride = 1;
+    s1.A = X; s1.strideA = node; s1.B = kw.T.d(); s1.strideB = node; s1.alpha = kw.wzd.d(); s1.alpha_stride = 1;
     dv.reduce(s1, nz, S1, 1.0, 0.0);
     GemmArgs s2{};   // S2 = sum wz x_i a_i
-    s2.A = kw.X.d(); s2.strideA = node; s2.opB = 1; s2.B = kw.A.d(); s2.strideB = node; s2.alpha = kw.wzd.d(); s2.alpha_stride = 1;
+    s2.A = X; s2.strideA = node; s2.opB = 1; s2.B = A; s2.strideB = node; s2.alpha = kw.wzd.d(); s2.alpha_stride = 1;
     dv.reduce(s2, nz, S2, 1.0, 0.0);
     GemmArgs nb{};   // T_i = R x_i
-    nb.A = st->R.d(); nb.B = kw.X.d(); nb.strideB = node; nb.C = kw.T.d(); nb.strideC = node;
+    nb.A = st->R.d(); nb.B = X; nb.strideB = node; nb.C = kw.T.d(); nb.strideC = node;
     dv.batched(nb, nz);
     GemmArgs s3{};   // S3 = sum wz a_i T_i
-    s3.opA = 1; s3.A = kw.A.d(); s3.strideA = node; s3.B = kw.T.d(); s3.strideB = node; s3.alpha = kw.wzd.d(); s3.alpha_stride = 1;
+    s3.opA = 1; s3.A = A; s3.strideA = node; s3.B = kw.T.d(); s3.strideB = node; s3.alpha = kw.wzd.d(); s3.alpha_stride = 1;
     dv.reduce(s3, nz, S3, 1.0, 0.0);
     GemmArgs s4{};   // S4 = sum wz a_i x_i
-    s4.opA = 1; s4.A = kw.A.d(); s4.strideA = node; s4.B = kw.X.d(); s4.strideB = node; s4.alpha = kw.wzd.d(); s4.alpha_stride = 1;
+    s4.opA = 1; s4.A = A; s4.strideA = node; s4.B = X; s4.strideB = node; s4.alpha = kw.wzd.d(); s4.alpha_stride = 1;
     dv.reduce(s4, nz, S4, 1.0, 0.0);
-    hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, kw.A.d(), node, kw.wd.d(), nz, dv.n(), 0.0, Sa);
+    hipLaunchKernelGGL(rc_lincomb, dim3(ceil_div(dv.n(), 64)), dim3(64), 0, ctx->stream, A, node, kw.wd.d(), nz, dv.n(), 0.0, Sa);
     ck(hipGetLastError());
     dv.herm(1, 1.0, Sa, a);                                  // a = sum w a_i
     dv.gemm(0, S2, 0, st->Rh.d(), x2);                       // m = S1 - S2 R^+
@@ -1957,6 +2030,9 @@ void kinetic_resolvent_dev(rcmps_context* ctx, const rcmps_state* st, int which,
     const size_t batch_bytes = (size_t)o.maxnodes * mb;
     ck(kw.X.ensure(batch_bytes));
     ck(kw.A.ensure(batch_bytes));
+    ck(kw.X2.ensure(batch_bytes));
+    ck(kw.A2.ensure(batch_bytes));
+    ck(kw.rhs2.ensure(2 * mb));
     ck(kw.LX.ensure(batch_bytes));
     ck(kw.T.ensure(batch_bytes));
     double* lev_c = kw.tmp.d() + 12 * mcount;    // current level covector / environment
@@ -1977,10 +2053,25 @@ void kinetic_resolvent_dev(rcmps_context* ctx, const rcmps_state* st, int which,
             ck(hipMemsetAsync(lev_c, 0, mb, ctx->stream));
             ck(hipMemsetAsync(lev_e, 0, mb, ctx->stream));
         }
+        // the node solutions of every (op, direction) the level needs, their bases grown together
+        std::vector<double> z(theta.size());
+        for (size_t i = 0; i < theta.size(); ++i) z[i] = std::cos(theta[i]);
+        double* Xs[2] = {kw.X.d(), kw.X2.d()};
+        double* As[2] = {kw.A.d(), kw.A2.d()};
+        std::vector<KinJob> jobs;
+        for (int comm = 0; comm < 2; ++comm) {
+            if (!(which & (1 << comm))) continue;
+            const double* opd = comm ? st->Ch.d() : st->Rh.d();
+            double* rhs = kw.rhs2.d() + (size_t)comm * mcount;
+            dv.gemm(0, st->rho.d(), 0, opd, rhs);   // forward solves on rho op^+
+            jobs.push_back(KinJob{&kw.basis[2 * comm], rhs, false, Xs[comm], 0, false});
+            if (gradient) jobs.push_back(KinJob{&kw.basis[2 * comm + 1], opd, true, As[comm], 0, false});   // on op^+
+        }
+        kinetic_solutions(ctx, st, dv, jobs, z);
         for (int comm = 0; comm < 2; ++comm) {
             if (!(which & (1 << comm))) continue;
             double v = 0;
-            kinetic_components(ctx, st, dv, comm, theta, w, gradient, v, c1, e1);
+            kinetic_components(ctx, st, dv, comm, theta, w, gradient, v, c1, e1, Xs[comm], As[comm]);
             val += v;
             if (gradient) {
                 dv.axpby(cd(1, 0), lev_c, cd(1, 0), c1, lev_c);

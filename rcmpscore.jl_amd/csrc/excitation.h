@@ -75,6 +75,14 @@ __global__ void rc_res_scale(long long N, int nz, int per_basis, int adjoint, co
 __global__ void rc_bcast_rhs(long long N, int nz, const double* src, long long ss, double* out);
 __global__ void rc_poisson_scale(long long N, int Dp, int D, int adjoint, double s, int k0, const double* mu,
                                  const double* Y, double* out);
+// up to four operands of rc_lindblad_multi, each with its direction
+struct RcLindbladMulti {
+    const double* X[4];
+    double* out[4];
+    int adj[4];
+};
+__global__ void rc_lindblad_multi(int Dp, const double* Q, const double* Qh, const double* R, const double* Rh,
+                                  RcLindbladMulti M);
 __global__ void rc_lindblad(int Dp, int adjoint, const double* Q, const double* Qh, const double* R, const double* Rh,
                             const double* X, double* out);
 __global__ void rc_two_sided(int Dp, int D, const double* A, const double* X, const double* B, double* out, int sylvester,

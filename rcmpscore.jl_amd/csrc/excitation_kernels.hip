@@ -813,11 +813,8 @@ hipError_t rc_launch_quartic_ode(dim3 grid, const RcOdeRun* dparams, hipStream_t
 // k-split over the waves into one accumulator, the two-sided term through the LDS column block
 // T = X R^+[:, bj] (adjoint X R[:, bj]).  Dp <= 64; out must not alias X.  grid (Dp/16)^2.
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) rc_lindblad(int Dp, int adjoint, const double* Q, const double* Qh,
-                                                   const double* R, const double* Rh, const double* X,
-                                                   double* out) {
-    __shared__ cplx red[1024];
-    __shared__ cplx T[64 * 16];
+__device__ __forceinline__ void lindblad_tile(int Dp, int adjoint, const double* Q, const double* Qh, const double* R,
+                                              const double* Rh, const double* X, double* out, cplx* red, cplx* T) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nbt = Dp / 16, bi = blockIdx.x / nbt, bj = blockIdx.x % nbt;
     const cplx* x = reinterpret_cast<const cplx*>(X);
@@ -833,6 +830,23 @@ __global__ void __launch_bounds__(256) rc_lindblad(int Dp, int adjoint, const do
     qprod_t(acc, l, T, Dp, bi, wave, lane, 1.0);
     const cplx v = qreduce(acc, red, wave, lane);
     reinterpret_cast<cplx*>(out)[(long long)(16 * bi + (tid >> 4)) * Dp + 16 * bj + (tid & 15)] = v;
+}
+
+__global__ void __launch_bounds__(256) rc_lindblad(int Dp, int adjoint, const double* Q, const double* Qh,
+                                                   const double* R, const double* Rh, const double* X,
+                                                   double* out) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T[64 * 16];
+    lindblad_tile(Dp, adjoint, Q, Qh, R, Rh, X, out, red, T);
+}
+
+// rc_lindblad on up to four operands at once (blockIdx.y = operand), each with its own direction
+__global__ void __launch_bounds__(256) rc_lindblad_multi(int Dp, const double* Q, const double* Qh, const double* R,
+                                                         const double* Rh, RcLindbladMulti M) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T[64 * 16];
+    const int b = blockIdx.y;
+    lindblad_tile(Dp, M.adj[b], Q, Qh, R, Rh, M.X[b], M.out[b], red, T);
 }
 
 // ---------------------------------------------------------------------------------------------

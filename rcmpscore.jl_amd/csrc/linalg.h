@@ -37,6 +37,23 @@ __global__ void rc_sylvester_tri(int Dp, int D, int adjoint, double s_re, double
 size_t rc_sylvester_tri_lds(int D);   // dynamic LDS bytes of rc_sylvester_tri (one workgroup of 256)
 #define RC_SYLVESTER_TRI_MAX_D 64
 __global__ void rc_dots(const double* V, long long stride, int n, const double* w, double* out);
+// Arnoldi steps of up to four bases in lockstep: basis b's step j orthogonalises w against V_0..V_j
+// (columns h1 and h2 of the two Gram-Schmidt passes, h0 = |w|^2 before and hn after them) and
+// writes V_{j+1}; vectors of n complex entries at stride n
+struct RcArnBasis {
+    double* V;
+    double* w;
+    const double* wl;   // w again, the left operand of |w|^2 (the products then compile as for any pair)
+    double *h0, *h1, *h2, *hn;
+    int j, pad;
+};
+struct RcArnBatch {
+    int n, nb;
+    RcArnBasis b[4];
+};
+__global__ void rc_arn_dots(RcArnBatch B, int pass);
+__global__ void rc_arn_subtract(RcArnBatch B, int pass);
+__global__ void rc_arn_normalize(RcArnBatch B);
 __global__ void rc_subtract_combo(const double* V, long long stride, int nk, const double* h, int n, double* w);
 __global__ void rc_add_combo(const double* V, long long stride, int nk, const double* y, int n, double* x);
 __global__ void rc_trace_prod(int Dp, const double* A, const double* B, double* out);

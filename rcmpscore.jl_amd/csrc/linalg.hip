@@ -487,14 +487,13 @@ __global__ void __launch_bounds__(256) rc_sylvester_tri(int Dp, int D, int adjoi
 
 size_t rc_sylvester_tri_lds(int D) { return (size_t)2 * D * D * sizeof(cplx); }
 
-// Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.  Thread t
-// adds elements t, t + 256, ... in increasing order, eight elements' loads issued together.
-__global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride, int n, const double* w,
-                                               double* out) {
-    __shared__ double sre[256], sim[256];
-    const int k = blockIdx.x, t = threadIdx.x;
-    const cplx* v = reinterpret_cast<const cplx*>(V) + (size_t)k * stride;
-    const cplx* ww = reinterpret_cast<const cplx*>(w);
+// One inner product sum_i conj(v[i]) w[i] by one workgroup into out[0..1].  Thread t adds elements
+// t, t + 256, ... in increasing order, eight elements' loads issued together.  Each term is
+// re v.re w.re + v.im w.im = fma(v.re, w.re, v.im w.im), im fma(v.re, w.im, -(v.im w.re)), written
+// out so that the rounding does not depend on which product the compiler fuses.
+__device__ __forceinline__ void dot_block(const cplx* v, const cplx* ww, int n, double* sre, double* sim,
+                                          double* out) {
+    const int t = threadIdx.x;
     double re = 0, im = 0;
     for (int i0 = t; i0 < n; i0 += 8 * 256) {
         cplx a[8], b[8];
@@ -507,17 +506,26 @@ __global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride
 #pragma unroll
         for (int u = 0; u < 8; ++u)
             if (i0 + 256 * u < n) {
-                re += a[u].re * b[u].re + a[u].im * b[u].im;
-                im += a[u].re * b[u].im - a[u].im * b[u].re;
+                const double pr = a[u].im * b[u].im, pi = a[u].im * b[u].re;
+                re += fma(a[u].re, b[u].re, pr);
+                im += fma(a[u].re, b[u].im, -pi);
             }
     }
-    tree_sum_256(re, im, sre, sim, out + 2 * k);
+    tree_sum_256(re, im, sre, sim, out);
 }
 
-// w -= sum_k h_k V_k   (h complex, host-provided array in device memory)
-__global__ void rc_subtract_combo(const double* V, long long stride, int nk, const double* h, int n, double* w) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.
+__global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride, int n, const double* w,
+                                               double* out) {
+    __shared__ double sre[256], sim[256];
+    const int k = blockIdx.x;
+    dot_block(reinterpret_cast<const cplx*>(V) + (size_t)k * stride, reinterpret_cast<const cplx*>(w), n, sre, sim,
+              out + 2 * k);
+}
+
+// element i of w -= sum_k h_k V_k
+__device__ __forceinline__ void subtract_elem(const double* V, long long stride, int nk, const double* h, int i,
+                                              double* w) {
     const cplx* v = reinterpret_cast<const cplx*>(V);
     const cplx* hh = reinterpret_cast<const cplx*>(h);
     cplx s = reinterpret_cast<cplx*>(w)[i];
@@ -535,6 +543,53 @@ __global__ void rc_subtract_combo(const double* V, long long stride, int nk, con
             if (k0 + u < nk) s = csub(s, cmul(c[u], x[u]));
     }
     reinterpret_cast<cplx*>(w)[i] = s;
+}
+
+// w -= sum_k h_k V_k   (h complex, host-provided array in device memory)
+__global__ void rc_subtract_combo(const double* V, long long stride, int nk, const double* h, int n, double* w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    subtract_elem(V, stride, nk, h, i, w);
+}
+
+// ---- Arnoldi steps of several bases in lockstep (blockIdx.y = basis); the arithmetic of each
+// basis is that of rc_dots, rc_subtract_combo and rc_normalize, so each gives the same bits as
+// its own step-by-step sequence.
+// pass 1: blocks 0..j: h1[k] = <V_k, w>, block j + 1: h0 = <w, w>; pass 2: blocks 0..j: h2[k] = <V_k, w>
+__global__ void __launch_bounds__(256) rc_arn_dots(RcArnBatch B, int pass) {
+    __shared__ double sre[256], sim[256];
+    const RcArnBasis& b = B.b[blockIdx.y];
+    const int k = blockIdx.x;
+    if (k > b.j + (pass == 1 ? 1 : 0)) return;
+    const cplx* w = reinterpret_cast<const cplx*>(b.w);
+    if (k == b.j + 1) {
+        dot_block(reinterpret_cast<const cplx*>(b.wl), w, B.n, sre, sim, b.h0);
+        return;
+    }
+    dot_block(reinterpret_cast<const cplx*>(b.V) + (size_t)k * B.n, w, B.n, sre, sim, (pass == 1 ? b.h1 : b.h2) + 2 * k);
+}
+
+// w -= sum_{k <= j} h_k V_k with the pass's projections
+__global__ void rc_arn_subtract(RcArnBatch B, int pass) {
+    const RcArnBasis& b = B.b[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    subtract_elem(b.V, B.n, b.j + 1, pass == 1 ? b.h1 : b.h2, i, b.w);
+}
+
+// hn = <w, w>, then V_{j+1} = w / sqrt(Re hn) (zero stays zero); one workgroup per basis
+__global__ void __launch_bounds__(256) rc_arn_normalize(RcArnBatch B) {
+    __shared__ double sre[256], sim[256];
+    __shared__ double s_n;
+    const RcArnBasis& b = B.b[blockIdx.y];
+    const cplx* w = reinterpret_cast<const cplx*>(b.w);
+    dot_block(reinterpret_cast<const cplx*>(b.wl), w, B.n, sre, sim, b.hn);
+    if (threadIdx.x == 0) s_n = b.hn[0];   // this thread's own store, read back
+    __syncthreads();
+    const double nrm2 = s_n;
+    const double sc = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 0.0;
+    cplx* out = reinterpret_cast<cplx*>(b.V) + (size_t)(b.j + 1) * B.n;
+    for (int i = threadIdx.x; i < B.n; i += 256) out[i] = cscale(sc, w[i]);
 }
 
 // x += sum_k y_k V_k
