@@ -141,8 +141,9 @@ def batch_main(args):
 def action_main(args):
     """One step = one Hamiltonian action F(W) on the reference's optimized Z2 checkpoint (D = 16, g = 2
     or D = 32, g = 2.7; tests/golden) at the settings of its GPU comparison runs: window 30, n_quad 512,
-    1e-13/1e-12, 256 theta nodes, p = 0.1, a seeded odd tangent.  Reports the kinetic and :phi^4: parts
-    apart as well."""
+    1e-13/1e-12, 256 theta nodes, p = 0.1, a seeded odd tangent.  D = 64 has no checkpoint among the
+    reference's fixtures: a seeded RandomStateZ2(64) (scaling 0.2, seed 42) at g = 2.7 stands in.
+    Reports the kinetic and :phi^4: parts apart as well."""
     import numpy as np
     import torch
 
@@ -151,15 +152,19 @@ def action_main(args):
     from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
     from rcmps_gpu.jld2 import read_jld2
     D = args.action
-    name = {16: "phi4_g2_D16_z2_state", 32: "phi4_g2.7_D32_z2_state"}.get(D)
+    name = {16: "phi4_g2_D16_z2_state", 32: "phi4_g2.7_D32_z2_state", 64: "RandomStateZ2(64)"}.get(D)
     if name is None:
-        print("bench.py: --action takes 16 or 32 (the reference's optimized checkpoints)", file=sys.stderr)
+        print("bench.py: --action takes 16 or 32 (the reference's optimized checkpoints) or 64", file=sys.stderr)
         return 2
-    rec = read_jld2(ROOT / "tests" / "golden" / f"{name}.jld2")
-    s = rcmps_gpu.State(rec.R, rec.K, steady_tol=1e-12)
+    if D == 64:
+        r, k = rcmps_gpu.random_state_z2_arrays(64, scaling=0.2, seed=42)
+        s, coupling = rcmps_gpu.State(r, k, steady_tol=1e-12), 2.7
+    else:
+        rec = read_jld2(ROOT / "tests" / "golden" / f"{name}.jld2")
+        s, coupling = rcmps_gpu.State(rec.R, rec.K, steady_tol=1e-12), rec.coupling
     s.integrator_param = IntegratorParameters(n_quad=512, max_point=30.0, atol=1e-13, rtol=1e-12)
     ctx = ExcitationContext(s, 0.1)
-    model = ExcitationModel("phi4", rec.coupling)
+    model = ExcitationModel("phi4", coupling)
     rng = np.random.default_rng(7)
     w = rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D))
     w[:D // 2, :D // 2] = 0
@@ -177,11 +182,14 @@ def action_main(args):
     sec, st = times["action"]
     out = {"metric": f"excitation Hamiltonian action wall time, D={D}", "value": sec, "unit": "s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": False,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": f"reference checkpoint {name}",
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": f"reference checkpoint {name}" if D != 64 else "seeded RandomStateZ2(64), scaling 0.2, seed 42, g=2.7",
            "config": {"workload": f"{name}, p=0.1, seeded odd tangent, window 30, n_quad 512, 1e-13/1e-12, "
                                   "256 theta nodes"},
            "phi4_part_s": times["phi4_part"][0], "kinetic_part_s": times["kinetic_part"][0],
-           "ode_ms": st["trajectory_ms"], "kinetic_actions": st["kinetic_actions"], "cpu_baseline": None}
+           "ode_ms": st["trajectory_ms"], "ode_rhs_calls": st["rhs_calls"][4],
+           "ode_us_per_rhs": 1e3 * st["trajectory_ms"] / max(1, st["rhs_calls"][4]),
+           "kinetic_actions": st["kinetic_actions"], "cpu_baseline": None}
     print(json.dumps(out), flush=True)
     s.close()
     return 0
@@ -256,7 +264,8 @@ def main(argv=None):
                     help="time the lockstep excitation dispersion (9 momenta, nev=2) instead")
     ap.add_argument("--exc-D", type=int, default=32)
     ap.add_argument("--action", type=int, default=0, metavar="D",
-                    help="time one excitation action on the reference's optimized D=16 or D=32 checkpoint")
+                    help="time one excitation action on the reference's optimized D=16 or D=32 checkpoint, or D=64 "
+                         "on a seeded RandomStateZ2(64)")
     ap.add_argument("--exc-chunk", type=int, default=128, help="tangents per coarse kinetic action call")
     args = ap.parse_args(argv)
     # the dispersion solve is minutes long: one timed step and no warmup unless asked for
