@@ -1793,75 +1793,78 @@ struct KinJob {
     const double* rhs;
     bool adj;
     double* X;
-    int want;      // the basis size this job waits for
+    int want;                // the basis size this job waits for
     bool done;
+    bool form = false;       // the projected solves passed: form and check the node solutions
+    bool terminal = false;
+    std::vector<cd> Y;       // projected solutions, k per node
 };
 
-// One pass of a job's test sequence: it returns done, or with the basis size it needs first.
-void kinetic_advance(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KinJob& jb, const std::vector<double>& z,
-                     int zhard) {
-    KineticWork& kw = ctx->kin;
-    const rcmps_kinetic_options& o = ctx->kopt;
+// The host part of one pass of a job's tests (kinetic_krylov.jl:114-155): the projected solves.  It
+// leaves the job asking for a larger basis, or with projected solutions to form (form), or failed
+// (neither: a terminal basis whose projected solves did not pass).
+void kinetic_project(const rcmps_kinetic_options& o, int maxdim, int initdim, KinJob& jb,
+                     const std::vector<double>& z, const std::vector<double>& zone) {
     KrylovBasis& b = *jb.b;
-    const int n = dv.n(), nz = (int)z.size();
-    const int maxdim = std::min(o.maxdim, st->D * st->D), initdim = std::min(o.initialdim, maxdim);
-    const std::vector<double> zone(1, z[zhard]);
-    std::vector<cd> Y;
-    std::vector<double> res(nz);
-    using clock = std::chrono::steady_clock;
-    auto since = [](clock::time_point t0) { return std::chrono::duration<double, std::milli>(clock::now() - t0).count(); };
-    auto project = [&](const std::vector<double>& zz, int threads, double& e) {
-        const auto t0 = clock::now();
-        const bool r = projected_solutions(b, zz, Y, e, threads);
-        kw.ms[1] += since(t0);
-        return r;
-    };
     jb.want = 0;
+    jb.form = false;
     if (b.k == 0) {
         jb.want = initdim;
         return;
     }
     const int k = b.k;
-    const bool terminal = b.exhausted || k >= maxdim;
+    jb.terminal = b.exhausted || k >= maxdim;
     double est = 0;
     // the node of largest z alone first: when its projected residual (one term of the full test)
     // misses the tolerance on a basis that can grow, the basis extends without the other nodes' solves
-    if (!terminal && nz > 1) {
-        const bool ok1 = project(zone, 1, est);
-        if (!ok1 || est > o.tolerance) {
+    if (!jb.terminal && z.size() > 1) {
+        if (!projected_solutions(b, zone, jb.Y, est, 1) || est > o.tolerance) {
             jb.want = std::min(2 * k, maxdim);
             return;
         }
     }
-    const bool ok = project(z, kProjectedThreads, est);
-    if (ok && (est <= o.tolerance || terminal)) {
-        const auto t0 = clock::now();
-        ck(kw.Yd.ensure(Y.size() * 16));
-        ck(hipMemcpyAsync(kw.Yd.p, Y.data(), Y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(rc_basis_combine, dim3(ceil_div(n, 256), ceil_div(nz, 8)), dim3(256), 0, ctx->stream,
-                           b.V.d(), (long long)n, k, kw.Yd.d(), nz, n, jb.X);
-        ck(hipGetLastError());
-        lindblad_batched(dv, st, jb.X, kw.LX.d(), kw.T.d(), nz, jb.adj);
-        hipLaunchKernelGGL(rc_shifted_residual, dim3(nz), dim3(256), 0, ctx->stream, jb.X, kw.LX.d(), jb.rhs,
-                           kw.zd.d(), n, kw.res.d());
-        ck(hipGetLastError());
-        ck(hipMemcpyAsync(res.data(), kw.res.d(), (size_t)nz * 8, hipMemcpyDeviceToHost, ctx->stream));
-        ck(hipStreamSynchronize(ctx->stream));
-        kw.ms[2] += since(t0);
-        double worst = 0;
-        for (double r2 : res) worst = std::max(worst, std::sqrt(r2) / b.beta);
-        const bool fallback = terminal && worst > o.tolerance && worst <= o.fallback_tolerance;
-        if (std::isfinite(worst) && (worst <= o.tolerance || fallback)) {
-            kw.max_res = std::max(kw.max_res, worst);
-            jb.done = true;
-            return;
-        }
-        if (b.exhausted) throw Fail{RCMPS_ERR_SOLVER};
+    const bool ok = projected_solutions(b, z, jb.Y, est, kProjectedThreads);
+    if (ok && (est <= o.tolerance || jb.terminal)) {
+        jb.form = true;
+        return;
     }
-    if (terminal) throw Fail{RCMPS_ERR_SOLVER};
+    if (!jb.terminal) jb.want = std::min(2 * k, maxdim);
+}
+
+// The device part: form the node solutions of a job whose projected solves passed and accept them on
+// their true relative residuals; otherwise the job asks for a larger basis (or fails when its basis
+// cannot grow).
+void kinetic_form(rcmps_context* ctx, const rcmps_state* st, Dev& dv, KinJob& jb, int nz, int maxdim) {
+    KineticWork& kw = ctx->kin;
+    const rcmps_kinetic_options& o = ctx->kopt;
+    KrylovBasis& b = *jb.b;
+    const int n = dv.n(), k = b.k;
+    ck(kw.Yd.ensure(jb.Y.size() * 16));
+    ck(hipMemcpyAsync(kw.Yd.p, jb.Y.data(), jb.Y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(rc_basis_combine, dim3(ceil_div(n, 256), ceil_div(nz, 8)), dim3(256), 0, ctx->stream, b.V.d(),
+                       (long long)n, k, kw.Yd.d(), nz, n, jb.X);
+    ck(hipGetLastError());
+    lindblad_batched(dv, st, jb.X, kw.LX.d(), kw.T.d(), nz, jb.adj);
+    hipLaunchKernelGGL(rc_shifted_residual, dim3(nz), dim3(256), 0, ctx->stream, jb.X, kw.LX.d(), jb.rhs, kw.zd.d(), n,
+                       kw.res.d());
+    ck(hipGetLastError());
+    std::vector<double> res(nz);
+    ck(hipMemcpyAsync(res.data(), kw.res.d(), (size_t)nz * 8, hipMemcpyDeviceToHost, ctx->stream));
+    ck(hipStreamSynchronize(ctx->stream));
+    double worst = 0;
+    for (double r2 : res) worst = std::max(worst, std::sqrt(r2) / b.beta);
+    const bool fallback = jb.terminal && worst > o.tolerance && worst <= o.fallback_tolerance;
+    if (std::isfinite(worst) && (worst <= o.tolerance || fallback)) {
+        kw.max_res = std::max(kw.max_res, worst);
+        jb.done = true;
+        return;
+    }
+    if (b.exhausted || jb.terminal) throw Fail{RCMPS_ERR_SOLVER};
     jb.want = std::min(2 * k, maxdim);
 }
 
+// Every round: the projected solves of all pending jobs at once (one host thread per job), then their
+// node solutions on the device one job after the other, then the extensions they ask for together.
 void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, std::vector<KinJob>& jobs,
                        const std::vector<double>& z) {
     KineticWork& kw = ctx->kin;
@@ -1885,23 +1888,54 @@ void kinetic_solutions(rcmps_context* ctx, const rcmps_state* st, Dev& dv, std::
     int zhard = 0;
     for (int i = 1; i < nz; ++i)
         if (std::fabs(z[i]) > std::fabs(z[zhard])) zhard = i;
+    const std::vector<double> zone(1, z[zhard]);
+    using clock = std::chrono::steady_clock;
+    auto since = [](clock::time_point t0) { return std::chrono::duration<double, std::milli>(clock::now() - t0).count(); };
     for (;;) {
+        std::vector<KinJob*> pending;
+        for (KinJob& jb : jobs)
+            if (!jb.done) pending.push_back(&jb);
+        if (pending.empty()) return;
+        auto t0 = clock::now();
+        if (pending.size() == 1) {
+            kinetic_project(o, maxdim, initdim, *pending[0], z, zone);
+        } else {
+            std::vector<std::thread> threads;
+            std::vector<std::exception_ptr> errs(pending.size());
+            for (size_t q = 0; q < pending.size(); ++q)
+                threads.emplace_back([&, q] {
+                    try {
+                        kinetic_project(o, maxdim, initdim, *pending[q], z, zone);
+                    } catch (...) {   // nothing may leave a host thread
+                        errs[q] = std::current_exception();
+                    }
+                });
+            for (auto& t : threads) t.join();
+            for (auto& e : errs)
+                if (e) std::rethrow_exception(e);
+        }
+        kw.ms[1] += since(t0);
         KrylovBasis* bs[4];
         bool adj[4];
         int tg[4], nw = 0;
-        for (KinJob& jb : jobs) {
-            if (jb.done) continue;
-            kinetic_advance(ctx, st, dv, jb, z, zhard);
-            if (jb.done) continue;
-            bs[nw] = jb.b;
-            adj[nw] = jb.adj;
-            tg[nw] = jb.want;
+        for (KinJob* jb : pending) {
+            if (jb->form) {
+                t0 = clock::now();
+                kinetic_form(ctx, st, dv, *jb, nz, maxdim);
+                kw.ms[2] += since(t0);
+            } else if (jb->want == 0) {
+                throw Fail{RCMPS_ERR_SOLVER};   // a terminal basis whose projected solves failed
+            }
+            if (jb->done) continue;
+            bs[nw] = jb->b;
+            adj[nw] = jb->adj;
+            tg[nw] = jb->want;
             ++nw;
         }
-        if (nw == 0) return;
-        const auto t0 = std::chrono::steady_clock::now();
+        if (nw == 0) continue;
+        t0 = clock::now();
         basis_extend_multi(ctx, st, dv, bs, adj, tg, nw);
-        kw.ms[0] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        kw.ms[0] += since(t0);
     }
 }
 
