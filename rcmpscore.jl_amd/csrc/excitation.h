@@ -146,3 +146,5 @@ __global__ void rc_ode_stage(const RcOdeCtl* c, long long N, long long half, dou
 __global__ void rc_trace_prod_batched(int Dp, const double* A, long long sa, const double* B, long long sb,
                                       double* out);
 __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, double* X);
+__global__ void rc_trace_axpy(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
+                              const double* Z, double* X);

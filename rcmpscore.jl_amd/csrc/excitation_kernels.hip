@@ -343,6 +343,27 @@ __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, doub
     x[i] = cadd(x[i], cmul(cmake(c_re, c_im), reinterpret_cast<const cplx*>(Z)[i]));
 }
 
+// X += c Z with c = -scale sum_j sum_k wz[k] tr[j nodes + k] over nj sets of `nodes` complex traces: the
+// sums in increasing j, then k, with each product and sum rounded on its own (the host loop's
+// arithmetic), made by every thread alike
+__global__ void rc_trace_axpy(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
+                              const double* Z, double* X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sr = 0.0, si = 0.0;
+    for (int j = 0; j < nj; ++j)
+        for (int k = 0; k < nodes; ++k) {
+            const double w = wz[k];
+            const double* t = tr + 2 * ((long long)j * nodes + k);
+            sr = __dadd_rn(sr, __dmul_rn(w, t[0]));
+            si = __dadd_rn(si, __dmul_rn(w, t[1]));
+        }
+    sr = __dmul_rn(sr, scale);
+    si = __dmul_rn(si, scale);
+    cplx* x = reinterpret_cast<cplx*>(X);
+    x[i] = cadd(x[i], cmul(cmake(-sr, -si), reinterpret_cast<const cplx*>(Z)[i]));
+}
+
 // ---------------------------------------------------------------------------------------------
 // One right-hand side of the :phi^4: action ODEs (forward and adjoint, every tangent and channel)
 // in a single launch.  Workgroup = one 16 x 16 output tile of one derivative matrix; all products
