@@ -9,8 +9,19 @@ struct RcStageCoef {
     int nk;
 };
 
-// arguments of rc_quartic_rhs (the state and derivative hold 4 blocks of 5 na matrices: forward x,
-// forward y, adjoint a0, adjoint a1; Vrep/Wrep hold v, w of tangent a at (5 a) Dp^2)
+// Layout of the :phi^4: action ODE state of na tangents (five channels each, RC_QCH): the forward x
+// channels, every tangent's forward y channels, every tangent's adjoint a0 channels, the adjoint a1
+// channels.  x and a1 have no tangent-dependent operand and the same start (rho, I) for every
+// tangent, so one copy serves all; rc_qmat gives the matrix index of block b, tangent a, channel ch
+// (nb = RC_QCH na), rc_qmats the count.  The forward half (x, y) comes first.
+#define RC_QCH 5
+__host__ __device__ inline int rc_qmat(int b, int a, int ch, int nb) {
+    return b == 0 ? ch : b == 1 ? RC_QCH + RC_QCH * a + ch : b == 2 ? RC_QCH + nb + RC_QCH * a + ch : RC_QCH + 2 * nb + ch;
+}
+__host__ __device__ inline int rc_qmats(int nb) { return 2 * RC_QCH + 2 * nb; }
+
+// arguments of rc_quartic_rhs (the state and derivative in the layout above, rc_qmats(5 na) matrices;
+// Vrep/Wrep hold v, w of tangent a at (5 a) Dp^2)
 // The epilogue chains the DOP853 stages: epi = 1 writes the next stage input
 // Unext = y + h (sum_{j<s} comb_j K_j + comb_s F) for this tile's elements (or the step end with
 // the B weights), epi = 2 writes this tile's error sums sum |E5.K/sc|^2, sum |E3.K/sc|^2 with

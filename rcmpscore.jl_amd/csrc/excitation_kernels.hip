@@ -513,9 +513,15 @@ __device__ __forceinline__ cplx quartic_tile(const RcQuarticArgs& g, const cplx*
                                              int bj, int mi, cplx* red, cplx* T1, cplx* T2) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int Dp = g.Dp;
-    const int nb = g.na * 5, blk = mi / nb, r = mi % nb, a = r / 5, k = r % 5;
+    const int nb = g.na * RC_QCH;
+    // block, tangent and channel of matrix mi (rc_qmat's inverse; the shared blocks 0 and 3 take a = 0)
+    int blk, a, k;
+    if (mi < RC_QCH) { blk = 0; a = 0; k = mi; }
+    else if (mi < RC_QCH + nb) { blk = 1; a = (mi - RC_QCH) / RC_QCH; k = (mi - RC_QCH) % RC_QCH; }
+    else if (mi < RC_QCH + 2 * nb) { blk = 2; a = (mi - RC_QCH - nb) / RC_QCH; k = (mi - RC_QCH - nb) % RC_QCH; }
+    else { blk = 3; a = 0; k = mi - RC_QCH - 2 * nb; }
     const long long m = (long long)Dp * Dp;
-    auto mat = [&](int b, int ch) { return U + ((long long)b * nb + a * 5 + ch) * m; };
+    auto mat = [&](int b, int ch) { return U + (long long)rc_qmat(b, a, ch, nb) * m; };
     const cplx* Q = reinterpret_cast<const cplx*>(g.Q);
     const cplx* Qh = reinterpret_cast<const cplx*>(g.Qh);
     const cplx* Qs = reinterpret_cast<const cplx*>(g.Qs) + (long long)a * g.qstride;     // tangent's momentum
@@ -530,7 +536,7 @@ __device__ __forceinline__ cplx quartic_tile(const RcQuarticArgs& g, const cplx*
     const int zt = g.zt, nk = Dp / 16, pw = g.wpar;
     const int px = k & 1, py = (k + 1 + pw) & 1;
     auto zm = [&](int pa, int pb) { return zmask(zt, nk, bi, bj, pa, pb); };
-    const QCoh qc{rc_rsrc(U, (uint32_t)(4LL * nb * m * (long long)sizeof(cplx))), U};
+    const QCoh qc{rc_rsrc(U, (uint32_t)((long long)rc_qmats(nb) * m * (long long)sizeof(cplx))), U};
     cacc acc;
     cacc_zero(acc);
     if (blk == 0) {
