@@ -107,6 +107,33 @@ __device__ __forceinline__ void tree_sum_256(double re, double im, double* sre, 
     }
 }
 
+// One inner product sum_i conj(v[i]) w[i] by one workgroup into out[0..1].  Thread t adds elements
+// t, t + 256, ... in increasing order, eight elements' loads issued together.  Each term is
+// re v.re w.re + v.im w.im = fma(v.re, w.re, v.im w.im), im fma(v.re, w.im, -(v.im w.re)), written
+// out so that the rounding does not depend on which product the compiler fuses.
+__device__ __forceinline__ void dot_block(const cplx* v, const cplx* ww, int n, double* sre, double* sim,
+                                          double* out) {
+    const int t = threadIdx.x;
+    double re = 0, im = 0;
+    for (int i0 = t; i0 < n; i0 += 8 * 256) {
+        cplx a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 256 * u < n) {
+                a[u] = v[i0 + 256 * u];
+                b[u] = ww[i0 + 256 * u];
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 256 * u < n) {
+                const double pr = a[u].im * b[u].im, pi = a[u].im * b[u].re;
+                re += fma(a[u].re, b[u].re, pr);
+                im += fma(a[u].re, b[u].im, -pi);
+            }
+    }
+    tree_sum_256(re, im, sre, sim, out);
+}
+
 // ---------------------------------------------------------------------------------------
 // FP64 MFMA 16x16x4.  Operand layout (one wave): A(i,k) at lane i + 16k, B(k,j) at lane
 // j + 16k; result element (4q + lane/16, lane%16) in component q of the v4d accumulator.

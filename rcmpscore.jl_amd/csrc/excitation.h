@@ -157,5 +157,15 @@ __global__ void rc_ode_stage(const RcOdeCtl* c, long long N, long long half, dou
 __global__ void rc_trace_prod_batched(int Dp, const double* A, long long sa, const double* B, long long sb,
                                       double* out);
 __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, double* X);
+__global__ void rc_sub_scaled_n(int n, const double* t, const double* Z, double* X, long long sx);
+__global__ void rc_add_identity_n(int Dp, int D, const double* t, double scale, double* X, long long sx);
+__global__ void rc_norms_n(int n, const double* Xl, const double* X, long long sx, double* out, long long so);
+__global__ void rc_poisson_scale_n(long long N, int Dp, int D, int adjoint, const double* s, int k0, const double* mu,
+                                   const double* Y, long long sy, double* out, long long so);
+__global__ void rc_axpby_n(int n, double ar, double ai, const double* X, long long sx, const double* b, const double* Y,
+                           long long sy, double* out, long long so);
+__global__ void rc_axpy_c_n(int n, const double* c, const double* Z, long long sz, double* X, long long sx);
+__global__ void rc_lindblad_n(int Dp, int adjoint, const double* Q, const double* Qh, const double* R, const double* Rh,
+                              const double* X, long long sx, double* out, long long so);
 __global__ void rc_trace_axpy(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
                               const double* Z, double* X);

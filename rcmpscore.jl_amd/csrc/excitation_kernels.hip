@@ -302,6 +302,77 @@ __global__ void rc_poisson_scale(long long N, int Dp, int D, int adjoint, double
     reinterpret_cast<cplx*>(out)[k] = cmake((y.re * den.re + y.im * den.im) / m, (y.im * den.re - y.re * den.im) / m);
 }
 
+// ---- the pinned Poisson solves and tangent algebra of many tangents at once (blockIdx.y = tangent
+// v, strides in complex elements); each kernel repeats its single-matrix counterpart's arithmetic.
+// X_v -= t_v Z with t_v the device complex at t + 2 v (rc_sub_scaled with den = 1)
+__global__ void rc_sub_scaled_n(int n, const double* t, const double* Z, double* X, long long sx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
+    if (i >= n) return;
+    const cplx tt = cmake(t[2 * v], t[2 * v + 1]);
+    const double dre = 1.0, dim = 0.0, m = dre * dre + dim * dim;
+    const cplx f = cmake((tt.re * dre + tt.im * dim) / m, (tt.im * dre - tt.re * dim) / m);
+    cplx* x = reinterpret_cast<cplx*>(X) + v * sx;
+    x[i] = csub(x[i], cmul(f, reinterpret_cast<const cplx*>(Z)[i]));
+}
+
+// X_v += scale t_v I on the D x D block (rc_add_identity)
+__global__ void rc_add_identity_n(int Dp, int D, const double* t, double scale, double* X, long long sx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
+    if (i >= D) return;
+    cplx* x = reinterpret_cast<cplx*>(X) + v * sx;
+    x[i * Dp + i].re += scale * t[2 * v];
+    x[i * Dp + i].im += scale * t[2 * v + 1];
+}
+
+// out + so v = <X_v, X_v>, one workgroup per tangent (Xl = X: the left operand passed apart, so the
+// products compile as rc_dots's do)
+__global__ void __launch_bounds__(256) rc_norms_n(int n, const double* Xl, const double* X, long long sx, double* out,
+                                                  long long so) {
+    __shared__ double sre[256], sim[256];
+    const int v = blockIdx.x;
+    dot_block(reinterpret_cast<const cplx*>(Xl) + v * sx, reinterpret_cast<const cplx*>(X) + v * sx, n, sre, sim,
+              out + so * v);
+}
+
+// rc_poisson_scale with the shift s_v of tangent v
+__global__ void rc_poisson_scale_n(long long N, int Dp, int D, int adjoint, const double* s, int k0, const double* mu,
+                                   const double* Y, long long sy, double* out, long long so) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (k >= N) return;
+    cplx* o = reinterpret_cast<cplx*>(out) + v * so;
+    if (k / Dp >= D || k % Dp >= D) {
+        o[k] = cmake(0, 0);
+        return;
+    }
+    cplx l = reinterpret_cast<const cplx*>(mu)[k];
+    if (adjoint) l = cconj(l);
+    const cplx den = cmake(-l.re + (k == k0 ? 1.0 : 0.0), -l.im + s[v]);
+    const cplx y = (reinterpret_cast<const cplx*>(Y) + v * sy)[k];
+    const double m = cabs2(den);
+    o[k] = cmake((y.re * den.re + y.im * den.im) / m, (y.im * den.re - y.re * den.im) / m);
+}
+
+// out_v = a X_v + b_v Y_v with b_v the device complex at b + 2 v (rc_axpby)
+__global__ void rc_axpby_n(int n, double ar, double ai, const double* X, long long sx, const double* b, const double* Y,
+                           long long sy, double* out, long long so) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
+    if (i >= n) return;
+    const cplx* x = reinterpret_cast<const cplx*>(X) + v * sx;
+    const cplx* y = reinterpret_cast<const cplx*>(Y) + v * sy;
+    cplx r = cmul(cmake(ar, ai), x[i]);
+    r = cadd(r, cmul(cmake(b[2 * v], b[2 * v + 1]), y[i]));
+    (reinterpret_cast<cplx*>(out) + v * so)[i] = r;
+}
+
+// X_v += c_v Z_v with c_v the device complex at c + 2 v (rc_axpy_c)
+__global__ void rc_axpy_c_n(int n, const double* c, const double* Z, long long sz, double* X, long long sx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
+    if (i >= n) return;
+    cplx* x = reinterpret_cast<cplx*>(X) + v * sx;
+    x[i] = cadd(x[i], cmul(cmake(c[2 * v], c[2 * v + 1]), (reinterpret_cast<const cplx*>(Z) + v * sz)[i]));
+}
+
 // out[v] = src[v / nz] (N complex each): block 1 of basis b's right-hand side for each node.
 // grid (nb nz, ceil(N / 256)).
 __global__ void rc_bcast_rhs(long long N, int nz, const double* src, long long ss, double* out) {
@@ -865,6 +936,16 @@ __global__ void __launch_bounds__(256) rc_lindblad(int Dp, int adjoint, const do
     __shared__ cplx red[1024];
     __shared__ cplx T[64 * 16];
     lindblad_tile(Dp, adjoint, Q, Qh, R, Rh, X, out, red, T);
+}
+
+// rc_lindblad on count operands X + v sx into out + v so (blockIdx.y = v), one direction
+__global__ void __launch_bounds__(256) rc_lindblad_n(int Dp, int adjoint, const double* Q, const double* Qh,
+                                                     const double* R, const double* Rh, const double* X, long long sx,
+                                                     double* out, long long so) {
+    __shared__ cplx red[1024];
+    __shared__ cplx T[64 * 16];
+    const long long v = blockIdx.y;
+    lindblad_tile(Dp, adjoint, Q, Qh, R, Rh, X + 2 * v * sx, out + 2 * v * so, red, T);
 }
 
 // rc_lindblad on up to four operands at once (blockIdx.y = operand), each with its own direction

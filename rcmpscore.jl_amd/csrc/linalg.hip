@@ -487,33 +487,6 @@ __global__ void __launch_bounds__(256) rc_sylvester_tri(int Dp, int D, int adjoi
 
 size_t rc_sylvester_tri_lds(int D) { return (size_t)2 * D * D * sizeof(cplx); }
 
-// One inner product sum_i conj(v[i]) w[i] by one workgroup into out[0..1].  Thread t adds elements
-// t, t + 256, ... in increasing order, eight elements' loads issued together.  Each term is
-// re v.re w.re + v.im w.im = fma(v.re, w.re, v.im w.im), im fma(v.re, w.im, -(v.im w.re)), written
-// out so that the rounding does not depend on which product the compiler fuses.
-__device__ __forceinline__ void dot_block(const cplx* v, const cplx* ww, int n, double* sre, double* sim,
-                                          double* out) {
-    const int t = threadIdx.x;
-    double re = 0, im = 0;
-    for (int i0 = t; i0 < n; i0 += 8 * 256) {
-        cplx a[8], b[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (i0 + 256 * u < n) {
-                a[u] = v[i0 + 256 * u];
-                b[u] = ww[i0 + 256 * u];
-            }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (i0 + 256 * u < n) {
-                const double pr = a[u].im * b[u].im, pi = a[u].im * b[u].re;
-                re += fma(a[u].re, b[u].re, pr);
-                im += fma(a[u].re, b[u].im, -pi);
-            }
-    }
-    tree_sum_256(re, im, sre, sim, out);
-}
-
 // Block-parallel inner products: out[k] = sum_i conj(V_k[i]) w[i], one workgroup per k.
 __global__ void __launch_bounds__(256) rc_dots(const double* V, long long stride, int n, const double* w,
                                                double* out) {
