@@ -169,3 +169,5 @@ __global__ void rc_lindblad_n(int Dp, int adjoint, const double* Q, const double
                               const double* X, long long sx, double* out, long long so);
 __global__ void rc_trace_axpy(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
                               const double* Z, double* X);
+__global__ void rc_trace_axpy_n(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
+                                const double* Z, long long sz, double* X, long long sx);

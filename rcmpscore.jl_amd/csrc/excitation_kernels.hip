@@ -417,10 +417,8 @@ __global__ void rc_axpy_c(int n, double c_re, double c_im, const double* Z, doub
 // X += c Z with c = -scale sum_j sum_k wz[k] tr[j nodes + k] over nj sets of `nodes` complex traces: the
 // sums in increasing j, then k, with each product and sum rounded on its own (the host loop's
 // arithmetic), made by every thread alike
-__global__ void rc_trace_axpy(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
-                              const double* Z, double* X) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void trace_axpy_elem(int nj, int nodes, const double* tr, const double* wz, double scale,
+                                                const double* Z, double* X, int i) {
     double sr = 0.0, si = 0.0;
     for (int j = 0; j < nj; ++j)
         for (int k = 0; k < nodes; ++k) {
@@ -433,6 +431,22 @@ __global__ void rc_trace_axpy(int n, int nj, int nodes, const double* tr, const 
     si = __dmul_rn(si, scale);
     cplx* x = reinterpret_cast<cplx*>(X);
     x[i] = cadd(x[i], cmul(cmake(-sr, -si), reinterpret_cast<const cplx*>(Z)[i]));
+}
+
+__global__ void rc_trace_axpy(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
+                              const double* Z, double* X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    trace_axpy_elem(nj, nodes, tr, wz, scale, Z, X, i);
+}
+
+// rc_trace_axpy for several tangents (blockIdx.y = v): traces tr + 2 v nj nodes, Z + v sz, X + v sx
+__global__ void rc_trace_axpy_n(int n, int nj, int nodes, const double* tr, const double* wz, double scale,
+                                const double* Z, long long sz, double* X, long long sx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long v = blockIdx.y;
+    trace_axpy_elem(nj, nodes, tr + 2 * v * nj * nodes, wz, scale, Z + 2 * v * sz, X + 2 * v * sx, i);
 }
 
 // ---------------------------------------------------------------------------------------------

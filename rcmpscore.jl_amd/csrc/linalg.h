@@ -23,8 +23,12 @@ __global__ void rc_zgemm_batched_w(GemmArgs g);
 __global__ void rc_zgemm_batched_lds(GemmArgs g);
 __global__ void rc_zgemm_reduce_partial(GemmArgs g, double* partial, int chunk);
 __global__ void rc_sum_partials(const double* partial, int nchunks, int n, double scale, double beta, double* C);
+__global__ void rc_sum_partials_seg(const double* partial, int nchunks, int n, double scale, double beta, double* C,
+                                    long long sC);
 __global__ void rc_lincomb(const double* X, long long stride, const double* coef, int ncoef, int n, double beta,
                            double* out);
+__global__ void rc_lincomb_n(const double* X, long long stride, long long sg, const double* coef, int ncoef, int n,
+                             double beta, double* out, long long so);
 __global__ void rc_axpby(int n, double ar, double ai, const double* X, double br, double bi, const double* Y,
                          double* out);
 __global__ void rc_herm(int Dp, int mode, double scale, const double* X, double* out);

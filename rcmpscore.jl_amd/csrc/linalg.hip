@@ -301,9 +301,8 @@ __global__ void __launch_bounds__(256) rc_zgemm_reduce_partial(GemmArgs g, doubl
 }
 
 // C = beta C + scale * sum_chunks partial (fixed order)
-__global__ void rc_sum_partials(const double* partial, int nchunks, int n, double scale, double beta, double* C) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void sum_partials_elem(const double* partial, int nchunks, int n, double scale,
+                                                  double beta, double* C, int i) {
     const cplx* p = reinterpret_cast<const cplx*>(partial);
     cplx s = cmake(0, 0);
     // eight chunks' loads in flight, added in increasing k
@@ -322,13 +321,26 @@ __global__ void rc_sum_partials(const double* partial, int nchunks, int n, doubl
     Cc[i] = out;
 }
 
+__global__ void rc_sum_partials(const double* partial, int nchunks, int n, double scale, double beta, double* C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sum_partials_elem(partial, nchunks, n, scale, beta, C, i);
+}
+
+// rc_sum_partials per segment (blockIdx.y = s): the nchunks partials of segment s into C + s sC
+__global__ void rc_sum_partials_seg(const double* partial, int nchunks, int n, double scale, double beta, double* C,
+                                    long long sC) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long sg = blockIdx.y;
+    sum_partials_elem(partial + 2 * sg * nchunks * n, nchunks, n, scale, beta, C + 2 * sg * sC, i);
+}
+
 // out = sum_k coef[k] * X_k   (X_k = X + k*stride), n complex elements; coef real
 // Sixteen terms' loads are issued together before they are added in order k = 0, 1, ..., so the
 // node sums of the pullback overlap their memory latency and keep the sequential sum's bits.
-__global__ void rc_lincomb(const double* X, long long stride, const double* coef, int ncoef, int n,
-                           double beta, double* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void lincomb_elem(const double* X, long long stride, const double* coef, int ncoef, double beta,
+                                             double* out, int i) {
     const cplx* x = reinterpret_cast<const cplx*>(X);
     cplx s = cmake(0, 0);
     int k = 0;
@@ -347,6 +359,22 @@ __global__ void rc_lincomb(const double* X, long long stride, const double* coef
     cplx* o = reinterpret_cast<cplx*>(out);
     if (beta != 0.0) s = cadd(s, cscale(beta, o[i]));
     o[i] = s;
+}
+
+__global__ void rc_lincomb(const double* X, long long stride, const double* coef, int ncoef, int n,
+                           double beta, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    lincomb_elem(X, stride, coef, ncoef, beta, out, i);
+}
+
+// rc_lincomb for several groups at once (blockIdx.y = g): X + g sg, out + g so (complex strides)
+__global__ void rc_lincomb_n(const double* X, long long stride, long long sg, const double* coef, int ncoef, int n,
+                             double beta, double* out, long long so) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long g = blockIdx.y;
+    lincomb_elem(X + 2 * g * sg, stride, coef, ncoef, beta, out + 2 * g * so, i);
 }
 
 // elementwise: out = a*X + b*Y (complex scalars)
