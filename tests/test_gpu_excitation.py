@@ -89,7 +89,9 @@ def test_action_parts_match_oracle(rg, d, seed, p, dense):
 def test_dense_transfer_resolvents_match_krylov(rg, name, D):
     """The kinetic part of F(W) for a batch of three tangents at two momenta on the reference's Z2
     checkpoints: resolvents from the transfer's eigendecomposition against the Krylov bases (solved
-    to 1e-11), within 1e-8 of |F|."""
+    to 1e-11), within 1e-8 of |F|; and the batch (its node sums made for all tangents together)
+    against each tangent alone with the dense resolvents, within 1e-12 of |F| (the pinned Poisson
+    products of the set-up take all right-hand sides at once, so the bits may differ)."""
     note(f"case: dense transfer vs Krylov resolvents, {name}")
     from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
     from rcmps_gpu.jld2 import read_jld2
@@ -105,6 +107,10 @@ def test_dense_transfer_resolvents_match_krylov(rg, name, D):
         t0 = time.perf_counter()
         fs = action_batch(ctxs, ws, model, 64, parts=1, krylov_options=None if dense else tight)
         out[dense] = (fs, time.perf_counter() - t0)
+        if dense:
+            for c, w, f in zip(ctxs, ws, fs):
+                (alone,) = action_batch(c, [w], model, 64, parts=1)
+                assert np.abs(alone - f).max() <= 1e-12 * np.abs(f).max()
         for c in ctxs:
             c.close()
     note(f"  kinetic part of 3 tangents: dense {out[True][1]:.3f} s, Krylov {out[False][1]:.3f} s, "
