@@ -747,9 +747,10 @@ def kinetic_preconditioners(ops, *, nodes=8, floor=None, chunk=64, progress=None
         if not np.isfinite(m).all():
             raise SolverFailure("nonfinite kinetic preconditioner")
         vals, vecs = np.linalg.eigh((m + m.conj().T) / 2)
+        vc, den = vecs.conj(), np.maximum(lower, vals)   # made once, not per application
 
-        def apply(r, vecs=vecs, vals=vals):
-            return vecs @ ((vecs.conj().T @ r) / np.maximum(lower, vals))
+        def apply(r, vecs=vecs, vc=vc, den=den):
+            return vecs @ ((vc.T @ r) / den)
         out.append(dict(apply=apply, matrix=m, values=vals, vectors=vecs,
                         skew=float(np.linalg.norm(m - m.conj().T) / max(1.0, np.linalg.norm(m))), floor=lower,
                         nodes=nodes))
