@@ -120,6 +120,35 @@ def test_dense_transfer_resolvents_match_krylov(rg, name, D):
     s.close()
 
 
+@pytest.mark.parametrize("nodes", [112, 130])
+def test_krylov_kinetic_batch_equals_single_tangents(rg, nodes):
+    """The kinetic part of F(W) on the Krylov route (D = 40, padded to 48) for two tangents at two
+    momenta in one call against each tangent alone, within 1e-8 of |F| (the lockstep bases of a batch
+    may take more steps than one tangent's; observed: the same bits).  Each tangent's node sums are one segment of a segmented
+    reduction in chunks of ceil(nodes / 56) nodes at this size: 112 nodes split evenly, 130 do not and
+    take one reduction per tangent."""
+    note(f"case: Krylov-route kinetic batch vs single tangents, D=40, {nodes} nodes")
+    from rcmps_gpu.excitation import ExcitationContext, ExcitationModel, action_batch
+    r, k = rg.random_state_arrays(40, scaling=0.2, seed=7)
+    s = rg.State(r, k, steady_tol=1e-12)
+    model = ExcitationModel("phi4", 2.0)
+    rng = np.random.default_rng(nodes)
+    ws = [rng.standard_normal((40, 40)) + 1j * rng.standard_normal((40, 40)) for _ in range(2)]
+    tight = rg.KineticKrylovOptions(tolerance=1e-11, fallback_tolerance=1e-11)
+    ctxs = [ExcitationContext(s, p) for p in (0.1, 0.6)]
+    t0 = time.perf_counter()
+    fs = action_batch(ctxs, ws, model, nodes, parts=1, krylov_options=tight)
+    note(f"  batch of 2: {time.perf_counter() - t0:.3f} s")
+    for c, w, f in zip(ctxs, ws, fs):
+        assert np.isfinite(f).all()
+        (alone,) = action_batch(c, [w], model, nodes, parts=1, krylov_options=tight)
+        note(f"  |batch - alone| / |F| = {np.abs(alone - f).max() / np.abs(f).max():.2e}")
+        assert np.abs(alone - f).max() <= 1e-8 * np.abs(f).max()
+    for c in ctxs:
+        c.close()
+    s.close()
+
+
 @pytest.mark.parametrize("d,seed,p", [(2, 42, 0.7), (4, 7, 0.0)])
 def test_connected_element_matches_oracle(rg, d, seed, p):
     """<Phi_p(u)|H - E0|Phi_p(w)> by the device pair environments, four-block ODE and kinetic Krylov
